@@ -1,0 +1,723 @@
+// Host side of the aship C ABI: plans, FP64 operator tables, launch sequences
+// and the per-step hipGraph.  Kernels live in kernels.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/aship.h"
+#include "kernels.h"
+
+using namespace ash;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess) return fail(ASH_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+bool line_supported(int n) { return n == 64 || n == 128 || n == 192 || n == 256 || n == 384; }
+bool col_supported(int m) { return m == 128 || m == 256 || m == 512 || m == 1024; }
+
+// Python's round(): half to even
+long py_round(double v) {
+  double r = std::nearbyint(v);  // default rounding mode is to-nearest-even
+  return (long)r;
+}
+
+// Tapers of ASS:82-110 in FP64.
+std::vector<double> taper(int N, int n, int profile) {
+  std::vector<double> v(N, 1.0);
+  for (int i = 0; i < N; ++i) {
+    double s = -1.0;
+    if (i < n) s = (double)(n - i - 1) / n;
+    if (i >= N - n) s = (double)(i - (N - n - 1)) / n;
+    if (s < 0.0) continue;
+    if (profile == ASH_PROFILE_WENDLAND) {
+      double w = 1.0 - 10 * s * s * s + 15 * s * s * s * s - 6 * s * s * s * s * s;
+      v[i] = std::fmin(std::fmax(w, 0.0), 1.0);
+    } else {
+      v[i] = 1.0 - s * s;
+    }
+  }
+  return v;
+}
+
+// (n-1)-normalised mean-subtracted grid (ASS:403-408), natural FFT order.
+std::vector<double> centred_natural(int n, double d, double extra_div) {
+  std::vector<double> k(n);
+  double mean = 0.0;
+  for (int j = 0; j < n; ++j) {
+    k[j] = (double)j / (n - 1) / d * 2 * M_PI / extra_div;
+    mean += k[j];
+  }
+  mean /= n;
+  for (int j = 0; j < n; ++j) k[j] -= mean;
+  std::vector<double> nat(n);
+  for (int i = 0; i < n; ++i) nat[i] = k[(i + n / 2) % n];
+  return nat;
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t n = 0;
+};
+
+}  // namespace
+
+struct ash_plan {
+  ash_desc d;
+  Geom g;
+  double N = 0.0;     // beta / (2 c0^3 rho0)
+  double dz = 0.0;
+  double lambda_char = 0.0;
+  int kt_nonzero = 0;
+  float2* spec = nullptr;
+  double* ibefore = nullptr;
+  int* d_row = nullptr;
+  unsigned* scratch_max = nullptr;
+  Tables T{};
+  std::vector<void*> allocs;
+  int64_t ws_bytes = 0;
+  bool loaded = false;
+  int host_row = -1;
+  // graph of a plain step
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  hipStream_t cap_stream = nullptr;
+  ash_step_io gkey{};
+  double gkey_dz = 0.0;
+  bool use_graph = true;
+};
+
+namespace {
+
+template <class T>
+int upload(ash_plan* p, const std::vector<T>& h, const T** out) {
+  void* d = nullptr;
+  HIP_TRY(hipMalloc(&d, h.size() * sizeof(T) + 16));
+  HIP_TRY(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  p->allocs.push_back(d);
+  p->ws_bytes += (int64_t)(h.size() * sizeof(T));
+  *out = (const T*)d;
+  return ASH_OK;
+}
+
+std::vector<float2> twiddles(int L) {
+  std::vector<float2> t(L);
+  for (int k = 0; k < L; ++k) {
+    double a = -2.0 * M_PI * (double)k / (double)L;
+    t[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  return t;
+}
+
+int build_tables(ash_plan* p) {
+  const ash_desc& d = p->d;
+  const int nX = d.nX, nY = d.nY, nT = d.nT, M = p->g.M, MP = p->g.MP;
+  int rc;
+  std::vector<double> kx = centred_natural(nX, d.dX, 1.0), ky = centred_natural(nY, d.dY, 1.0);
+  if ((rc = upload(p, kx, &p->T.kx))) return rc;
+  if ((rc = upload(p, ky, &p->T.ky))) return rc;
+  // ASM t-wavenumber of the kept (positive) bins: fftshifted index (m + nT/2) % nT
+  std::vector<double> ktc(nT);
+  {
+    double mean = 0.0;
+    for (int j = 0; j < nT; ++j) { ktc[j] = (double)j / (nT - 1) / d.dT * 2 * M_PI / d.c0; mean += ktc[j]; }
+    mean /= nT;
+    double sabs = 0.0;
+    int cnt = 0;
+    for (int j = 0; j < nT; ++j) {
+      ktc[j] -= mean;
+      if (ktc[j] != 0.0) { sabs += std::fabs(ktc[j]); ++cnt; }
+    }
+    p->kt_nonzero = cnt;
+    p->lambda_char = cnt ? d.c0 / (sabs / cnt) : d.c0 / (1.0 / d.dT);
+  }
+  std::vector<double> kt(MP, 0.0), ka(MP, 0.0), al(MP, 0.0), as(MP, 0.0);
+  const int kept_hi = (nT + 1) / 2 - 1;  // natural bins 1..ceil(nT/2)-1 are kept
+  for (int m = 1; m <= kept_hi && m < MP; ++m) kt[m] = ktc[(m + nT / 2) % nT];
+  // attenuation (ASS:523-592)
+  const double cl = 20.0 * std::log10(M_E);
+  std::vector<double> f(nT);
+  for (int m = 0; m < nT; ++m) f[m] = (double)m / (nT * d.dT);
+  if (d.alpha0 < 0) {
+    const double conv = 2.17e-3 / 1e12 * 1e2 / cl;
+    for (int m = 0; m <= M; ++m) {
+      al[m] = conv * f[m] * f[m];
+      const double a0 = (conv / std::pow(2 * M_PI, 2)) * std::tan(M_PI) * ((2 * M_PI * f[m]) - (2 * M_PI * d.f0));
+      as[m] = m == 0 ? 0.0 : 2 * M_PI * a0 * f[m];
+    }
+  } else {
+    const double pw = d.attenPow;
+    const double conv = d.alpha0 / std::pow(1e6, pw) * 1e2 / cl;
+    const bool logb = std::fmod(pw, 2.0) == 1.0;
+    for (int m = 0; m <= M; ++m) {
+      al[m] = conv * std::pow(f[m], pw);
+      double a0;
+      if (logb) {
+        const double fs = m == 0 ? (nT > 1 ? f[1] / 2 : 0.0) : f[m];
+        a0 = (-2 * conv / std::pow(2 * M_PI, pw) / M_PI) * (std::log(2 * M_PI * fs) - std::log(2 * M_PI * d.f0));
+      } else {
+        a0 = (conv / std::pow(2 * M_PI, pw)) * std::tan(M_PI * pw / 2) *
+             (std::pow(2 * M_PI * f[m], pw - 1) - std::pow(2 * M_PI * d.f0, pw - 1));
+      }
+      as[m] = m == 0 ? 0.0 : 2 * M_PI * a0 * f[m];
+    }
+  }
+  for (int m = 0; m <= M; ++m) ka[m] = 2 * M_PI * f[m] / d.c0;
+  if ((rc = upload(p, kt, &p->T.kt))) return rc;
+  if ((rc = upload(p, ka, &p->T.ka))) return rc;
+  if ((rc = upload(p, al, &p->T.alpha))) return rc;
+  if ((rc = upload(p, as, &p->T.astar))) return rc;
+  // boundary tapers (ASS:116-131)
+  const long bx = std::max(py_round(nX * d.boundaryFactor), 1L);
+  const long by = std::max(py_round(nY * d.boundaryFactor), 1L);
+  const long bt = std::max(py_round(nT * d.boundaryFactor), 1L);
+  std::vector<double> vx(nX, 1.0), vy(nY, 1.0), vt(nT, 1.0);
+  if (d.useBoundaryLayer) {
+    vx = taper(nX, (int)bx, d.boundaryProfile);
+    vy = taper(nY, (int)by, d.boundaryProfile);
+    vt = taper(nT, (int)bt, d.boundaryProfile);
+  }
+  auto tof = [](const std::vector<double>& v, bool sq) {
+    std::vector<float> o(v.size());
+    for (size_t i = 0; i < v.size(); ++i) o[i] = (float)(sq ? std::sqrt(v[i]) : v[i]);
+    return o;
+  };
+  if ((rc = upload(p, tof(vx, true), &p->T.tapx_h))) return rc;
+  if ((rc = upload(p, tof(vy, true), &p->T.tapy_h))) return rc;
+  if ((rc = upload(p, tof(vt, true), &p->T.tapt_h))) return rc;
+  if ((rc = upload(p, tof(vx, false), &p->T.tapx_f))) return rc;
+  if ((rc = upload(p, tof(vy, false), &p->T.tapy_f))) return rc;
+  if ((rc = upload(p, tof(vt, false), &p->T.tapt_f))) return rc;
+  // frequency-weighted mask (ASS:144-179): always from the profile tapers
+  {
+    std::vector<double> wx = taper(nX, (int)bx, d.boundaryProfile), wy = taper(nY, (int)by, d.boundaryProfile);
+    std::vector<float> lx(nX), ly(nY);
+    for (int i = 0; i < nX; ++i) lx[i] = (float)std::log2(wx[i]);
+    for (int i = 0; i < nY; ++i) ly[i] = (float)std::log2(wy[i]);
+    std::vector<float> fe(MP, 0.f);
+    const double f1 = M >= 1 ? 1.0 / (nT * d.dT) : 1.0;
+    for (int m = 0; m <= M; ++m) {
+      const double fm = std::fmax((double)m / (nT * d.dT), f1);
+      fe[m] = (float)std::fmin(std::fmax(d.f0 / fm, 0.1), 10.0);
+    }
+    if ((rc = upload(p, lx, &p->T.lgx))) return rc;
+    if ((rc = upload(p, ly, &p->T.lgy))) return rc;
+    if ((rc = upload(p, fe, &p->T.fw_exp))) return rc;
+  }
+  // super-absorbing weights (ASS:275-284) and c0/omega (ASS:219-228)
+  {
+    auto ramp = [](int N, int n) {
+      std::vector<float> w(N, 0.f);
+      for (int i = 0; i < N; ++i) {
+        if (i < n) w[i] = (float)((double)(n - i - 1) / n);
+        if (i >= N - n) w[i] = (float)((double)(i - (N - n - 1)) / n);
+      }
+      return w;
+    };
+    if ((rc = upload(p, ramp(nX, (int)bx), &p->T.sa_wx))) return rc;
+    if ((rc = upload(p, ramp(nY, (int)by), &p->T.sa_wy))) return rc;
+    std::vector<float> cw(MP, 0.f);
+    const double w1 = 2 * M_PI * (1.0 / (nT * d.dT));
+    for (int m = 1; m <= M; ++m) {
+      const double w = 2 * M_PI * ((double)m / (nT * d.dT));
+      if (std::fabs(w) > 0.5 * w1) cw[m] = (float)(d.c0 / w);
+    }
+    if ((rc = upload(p, cw, &p->T.sa_cw))) return rc;
+  }
+  if ((rc = upload(p, twiddles(nX), &p->T.twx))) return rc;
+  if ((rc = upload(p, twiddles(nY), &p->T.twy))) return rc;
+  if ((rc = upload(p, twiddles(M), &p->T.twm))) return rc;
+  {
+    std::vector<float2> tn(M + 1);
+    for (int k = 0; k <= M; ++k) {
+      double a = -2.0 * M_PI * (double)k / (double)nT;
+      tn[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    if ((rc = upload(p, tn, &p->T.twn))) return rc;
+  }
+  return ASH_OK;
+}
+
+// transverse cosine filter band of ASS:411-426 for step dz
+void filter_band(const ash_plan* p, double dz, double* k_start, double* k_max) {
+  const ash_desc& d = p->d;
+  const double kmax = std::min(M_PI / d.dX, M_PI / d.dY);
+  const double norm_step = dz / p->lambda_char;
+  if (d.useAdaptiveFiltering) {
+    const double cutoff = d.freqFilterThreshold * (1 + d.adaptiveFilterStrength * norm_step * 5);
+    *k_max = std::min(kmax * (1 - cutoff), kmax * 0.95);
+    *k_start = *k_max * 0.8;
+  } else {
+    *k_max = kmax * 0.95;
+    *k_start = *k_max * 0.9;
+  }
+}
+
+OpDesc make_op(const ash_plan* p, bool prop, double dz_prop, bool att, double dz_att, double dz_filter) {
+  OpDesc o{};
+  o.use_prop = prop;
+  o.use_att = att;
+  o.obliquity = p->d.useObliquityCorrection;
+  o.clip_negative = p->d.alpha0 < 0;
+  o.dz_prop = dz_prop;
+  o.dz_att = dz_att;
+  filter_band(p, dz_filter, &o.k_start, &o.k_max);
+  o.norm = (float)(1.0 / ((double)p->d.nX * p->d.nY));
+  return o;
+}
+
+// ---- launch helpers --------------------------------------------------------
+template <int L>
+void launch_xfwd_L(ash_plan* p, hipStream_t s) {
+  hipLaunchKernelGGL(k_xfft_fwd<L>, dim3(p->g.nY * p->g.nfb), dim3(LNT), 0, s, p->spec, p->T, p->g);
+}
+template <int L>
+void launch_ypass_L(ash_plan* p, const OpDesc& op, hipStream_t s) {
+  hipLaunchKernelGGL(k_ypass<L>, dim3(p->g.nX * p->g.nfb), dim3(LNT), 0, s, p->spec, p->T, p->g, op);
+}
+template <int L, int EPI>
+void launch_xinv_L(ash_plan* p, const EpiDesc& e, hipStream_t s) {
+  hipLaunchKernelGGL((k_xfft_inv<L, EPI>), dim3(p->g.nY * p->g.nfb), dim3(LNT), 0, s, p->spec, p->T, p->g, e);
+}
+template <int L>
+void launch_ysa_L(ash_plan* p, const ScreenDesc& sd, hipStream_t s) {
+  hipLaunchKernelGGL(k_ysa<L>, dim3(p->g.nX * p->g.nfb), dim3(LNT), 0, s, p->spec, p->T, p->g, sd);
+}
+template <int M>
+void launch_col_M(ash_plan* p, const ColDesc& cd, hipStream_t s) {
+  const int ncols = p->g.nX * p->g.nY;
+  hipLaunchKernelGGL(k_column<M>, dim3((ncols + CPB - 1) / CPB), dim3(CNT), 0, s, p->spec, p->T, p->g, cd);
+}
+
+#define DISPATCH_L(n, CALL)                    \
+  switch (n) {                                 \
+    case 64: { constexpr int L = 64; CALL; } break;   \
+    case 128: { constexpr int L = 128; CALL; } break; \
+    case 192: { constexpr int L = 192; CALL; } break; \
+    case 256: { constexpr int L = 256; CALL; } break; \
+    case 384: { constexpr int L = 384; CALL; } break; \
+    default: break;                            \
+  }
+
+void xfwd(ash_plan* p, hipStream_t s) { DISPATCH_L(p->g.nX, launch_xfwd_L<L>(p, s)); }
+void ypass(ash_plan* p, const OpDesc& op, hipStream_t s) { DISPATCH_L(p->g.nY, launch_ypass_L<L>(p, op, s)); }
+void xinv(ash_plan* p, int epi, const EpiDesc& e, hipStream_t s) {
+  if (epi == 0) { DISPATCH_L(p->g.nX, (launch_xinv_L<L, 0>(p, e, s))); }
+  else if (epi == 1) { DISPATCH_L(p->g.nX, (launch_xinv_L<L, 1>(p, e, s))); }
+  else { DISPATCH_L(p->g.nX, (launch_xinv_L<L, 2>(p, e, s))); }
+}
+void ysa(ash_plan* p, const ScreenDesc& sd, hipStream_t s) { DISPATCH_L(p->g.nY, launch_ysa_L<L>(p, sd, s)); }
+void column(ash_plan* p, const ColDesc& cd, hipStream_t s) {
+  switch (p->g.M) {
+    case 128: launch_col_M<128>(p, cd, s); break;
+    case 256: launch_col_M<256>(p, cd, s); break;
+    case 512: launch_col_M<512>(p, cd, s); break;
+    case 1024: launch_col_M<1024>(p, cd, s); break;
+    default: break;
+  }
+}
+
+ColDesc col_base(const ash_plan* p) {
+  ColDesc c{};
+  c.N = p->N;
+  c.dT = p->d.dT;
+  c.dz = p->dz;
+  c.cfl_target = (float)p->d.cflTarget;
+  c.subcycle = 0;
+  c.cax = (p->d.nX / 2) * p->d.nY + p->d.nY / 2;
+  return c;
+}
+
+int nl_kind(const ash_plan* p) {
+  if (!p->d.useSplitStep) return NL_RUSANOV;
+  if (p->d.fluxScheme == 1) return NL_KT;
+  return p->d.useTVD ? NL_TVD : NL_RUSANOV;
+}
+
+double beta_tvd(const ash_plan* p, double dz) {
+  const double norm_step = dz / (p->d.c0 / p->d.f0);
+  return std::max(1.0, 2.0 - p->d.adaptiveFilterStrength * norm_step * 10);
+}
+
+EpiDesc epi_base(const ash_plan* p) {
+  EpiDesc e{};
+  e.sa_strength = (float)p->d.superAbsorbingStrength;
+  e.inv2dx = (float)(1.0 / (2 * p->d.dX));
+  e.invdx = (float)(1.0 / p->d.dX);
+  e.scale = 1.f;
+  return e;
+}
+
+// The composite march step (ASS:874-913) on the spectral state, leaving the
+// state after the last attenuation visit with epilogue `epi_last` fused.
+void enqueue_composite(ash_plan* p, double dz, int epi_last, const EpiDesc& elast, hipStream_t s) {
+  EpiDesc e = epi_base(p);
+  ColDesc cd = col_base(p);
+  cd.dz = dz;
+  cd.beta_tvd = (float)beta_tvd(p, dz);
+  cd.nl = nl_kind(p);
+  cd.subcycle = p->d.subcycle;
+  if (p->d.useSplitStep) {
+    // attenuation half + propagator half, merged (both diagonal in k-space)
+    xfwd(p, s);
+    ypass(p, make_op(p, true, dz / 2, true, dz * 0.5, dz), s);
+    e.half = 1;
+    xinv(p, 1, e, s);
+    cd.mode = COL_NL;
+    cd.taper = 1;
+    column(p, cd, s);
+    xfwd(p, s);
+    ypass(p, make_op(p, true, dz / 2, false, 0.0, dz), s);
+    xinv(p, 1, e, s);
+    cd.mode = COL_TAPER;
+    cd.nl = NL_NONE;
+    column(p, cd, s);
+    xfwd(p, s);
+    ypass(p, make_op(p, false, 0.0, true, dz * 0.5, dz), s);
+    xinv(p, epi_last, elast, s);
+  } else {
+    xfwd(p, s);
+    ypass(p, make_op(p, true, dz, false, 0.0, dz), s);
+    e.half = 0;
+    xinv(p, 1, e, s);
+    cd.mode = COL_NL;
+    cd.taper = 2;
+    column(p, cd, s);
+    xfwd(p, s);
+    ypass(p, make_op(p, false, 0.0, true, dz * 1.0, dz), s);
+    xinv(p, epi_last, elast, s);
+  }
+}
+
+void enqueue_step(ash_plan* p, const ash_step_io* io, hipStream_t s) {
+  EpiDesc el = epi_base(p);
+  el.fw = p->d.useFreqWeightedBoundary;
+  el.sa = p->d.useSuperAbsorbing;
+  enqueue_composite(p, io->dz, 2, el, s);
+  ScreenDesc sd{};
+  sd.sa_strength = (float)p->d.superAbsorbingStrength;
+  sd.inv2dy = (float)(1.0 / (2 * p->d.dY));
+  sd.invdy = (float)(1.0 / p->d.dY);
+  const double f0_bin = p->d.f0 / (1.0 / (p->d.nT * p->d.dT));
+  sd.f0b = (float)std::fmax(f0_bin, 1.0);
+  if (p->d.useSuperAbsorbing || io->n_screens > 0) {
+    sd.sa = p->d.useSuperAbsorbing;
+    if (io->n_screens > 0) { sd.phase = io->screen_phase_dev[0]; sd.amp = io->screen_amp_dev[0]; }
+    ysa(p, sd, s);
+    for (int k = 1; k < io->n_screens; ++k) {
+      ScreenDesc s2 = sd;
+      s2.sa = 0;
+      s2.phase = io->screen_phase_dev[k];
+      s2.amp = io->screen_amp_dev[k];
+      ysa(p, s2, s);
+    }
+  }
+  ColDesc cd = col_base(p);
+  cd.mode = COL_FINAL;
+  cd.row = p->d_row;
+  cd.max_rows = io->max_rows;
+  cd.pnp = io->pnp_dev; cd.ppp = io->ppp_dev; cd.pI = io->pI_dev; cd.pIloss = io->pIloss_dev; cd.pax = io->pax_dev;
+  cd.stats = io->stats_dev;
+  cd.ibefore = p->ibefore;
+  cd.rout = io->field_out_dev;
+  cd.n_src = io->n_src_dense;
+  for (int k = 0; k < io->n_src_dense && k < 4; ++k) cd.src[k] = io->src_dense_dev[k];
+  cd.smask = io->src_mask_dev;
+  cd.sdelay = io->src_delay_dev;
+  cd.taxis = io->taxis_dev;
+  cd.s_f0 = io->src_f0; cd.s_p0 = io->src_p0; cd.s_ncyc = io->src_ncycles; cd.s_dur = io->src_dur;
+  column(p, cd, s);
+  hipLaunchKernelGGL(k_advance_row, dim3(1), dim3(1), 0, s, p->d_row);
+}
+
+int load_real(ash_plan* p, const float* in, uint32_t* stats, double* ib, hipStream_t s) {
+  ColDesc cd = col_base(p);
+  cd.mode = COL_LOAD;
+  cd.rin = in;
+  cd.stats = stats;
+  cd.ibefore = ib;
+  column(p, cd, s);
+  return ASH_OK;
+}
+
+int store_real(ash_plan* p, float* out, hipStream_t s) {
+  ColDesc cd = col_base(p);
+  cd.mode = COL_STORE;
+  cd.rout = out;
+  column(p, cd, s);
+  return ASH_OK;
+}
+
+int check_launch() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(ASH_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+  return ASH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ash_abi_version(void) { return ASH_ABI_VERSION; }
+
+const char* ash_last_error(void) { return g_err.c_str(); }
+
+int ash_plan_create(const ash_desc* desc, ash_plan** out) {
+  if (!desc || !out) return fail(ASH_E_ARG, "null argument");
+  *out = nullptr;
+  const ash_desc& d = *desc;
+  if (d.nX <= 0 || d.nY <= 0 || d.nT <= 0) return fail(ASH_E_ARG, "grid sizes must be positive");
+  if (!(d.dX > 0 && d.dY > 0 && d.dT > 0 && d.c0 > 0 && d.rho0 > 0))
+    return fail(ASH_E_ARG, "dX, dY, dT, c0 and rho0 must be positive");
+  if (!line_supported(d.nX) || !line_supported(d.nY))
+    return fail(ASH_E_UNSUPPORTED, "nX and nY must be one of 64, 128, 192, 256, 384");
+  if (d.nT % 2 != 0 || !col_supported(d.nT / 2))
+    return fail(ASH_E_UNSUPPORTED, "nT must be one of 256, 512, 1024, 2048");
+  ash_plan* p = new ash_plan();
+  p->d = d;
+  if (p->d.cflTarget <= 0) p->d.cflTarget = 0.5;
+  p->g.nX = d.nX; p->g.nY = d.nY; p->g.nT = d.nT;
+  p->g.M = d.nT / 2;
+  p->g.MP = ((p->g.M + 1 + LF - 1) / LF) * LF;
+  p->g.nfb = p->g.MP / LF;
+  p->N = d.beta / (2 * d.c0 * d.c0 * d.c0 * d.rho0);
+  p->dz = 0.0;
+  int rc = build_tables(p);
+  if (rc) { ash_plan_destroy(p); return rc; }
+  const size_t nspec = (size_t)d.nX * d.nY * p->g.MP;
+  hipError_t e;
+  if ((e = hipMalloc(&p->spec, nspec * sizeof(float2))) != hipSuccess ||
+      (e = hipMalloc(&p->ibefore, (size_t)d.nX * d.nY * sizeof(double))) != hipSuccess ||
+      (e = hipMalloc(&p->d_row, 64)) != hipSuccess || (e = hipMalloc(&p->scratch_max, 64)) != hipSuccess) {
+    ash_plan_destroy(p);
+    return fail(ASH_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
+  p->allocs.push_back(p->spec);
+  p->allocs.push_back(p->ibefore);
+  p->allocs.push_back(p->d_row);
+  p->allocs.push_back(p->scratch_max);
+  // padding bins are read (masked) but never written: clear once
+  if ((e = hipMemset(p->spec, 0, nspec * sizeof(float2))) != hipSuccess ||
+      (e = hipMemset(p->d_row, 0, 64)) != hipSuccess || (e = hipDeviceSynchronize()) != hipSuccess) {
+    ash_plan_destroy(p);
+    return fail(ASH_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
+  }
+  p->ws_bytes += (int64_t)(nspec * sizeof(float2) + (size_t)d.nX * d.nY * sizeof(double));
+  *out = p;
+  return ASH_OK;
+}
+
+void ash_plan_destroy(ash_plan* p) {
+  if (!p) return;
+  if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
+  for (void* a : p->allocs) (void)hipFree(a);
+  delete p;
+}
+
+int ash_plan_set_dz(ash_plan* p, double dz) {
+  if (!p) return fail(ASH_E_ARG, "null plan");
+  if (!(dz > 0)) return fail(ASH_E_ARG, "dz must be positive");
+  p->dz = dz;
+  return ASH_OK;
+}
+
+int ash_load_field(ash_plan* p, const float* field, uint32_t* stats, void* stream) {
+  if (!p || !field) return fail(ASH_E_ARG, "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  load_real(p, field, stats, p->ibefore, s);
+  p->loaded = true;
+  return check_launch();
+}
+
+int ash_store_field(ash_plan* p, float* field, void* stream) {
+  if (!p || !field) return fail(ASH_E_ARG, "null argument");
+  if (!p->loaded) return fail(ASH_E_STATE, "no field loaded");
+  store_real(p, field, (hipStream_t)stream);
+  return check_launch();
+}
+
+int ash_step(ash_plan* p, const ash_step_io* io, void* stream) {
+  if (!p || !io) return fail(ASH_E_ARG, "null argument");
+  if (!p->loaded) return fail(ASH_E_STATE, "ash_step before ash_load_field");
+  if (!(io->dz > 0)) return fail(ASH_E_ARG, "dz must be positive");
+  if (io->row < 0 || io->row >= io->max_rows) return fail(ASH_E_ARG, "row out of range");
+  if (!io->pnp_dev || !io->ppp_dev || !io->pI_dev || !io->pIloss_dev || !io->pax_dev || !io->stats_dev)
+    return fail(ASH_E_ARG, "output buffers must be non-null");
+  if (io->n_screens < 0 || io->n_screens > 4 || io->n_src_dense < 0 || io->n_src_dense > 4)
+    return fail(ASH_E_ARG, "at most 4 screens and 4 dense sources per step");
+  if (io->src_mask_dev && (!io->src_delay_dev || !io->taxis_dev))
+    return fail(ASH_E_ARG, "sparse source needs delay and taxis arrays");
+  hipStream_t s = (hipStream_t)stream;
+  if (io->row != p->host_row) {
+    hipLaunchKernelGGL(k_set_row, dim3(1), dim3(1), 0, s, p->d_row, io->row);
+  }
+  const bool plain = io->n_screens == 0 && io->n_src_dense == 0 && !io->src_mask_dev && io->dz == p->dz;
+  if (plain && p->use_graph) {
+    const bool same = p->gexec && p->gkey.pnp_dev == io->pnp_dev && p->gkey.ppp_dev == io->ppp_dev &&
+                      p->gkey.pI_dev == io->pI_dev && p->gkey.pIloss_dev == io->pIloss_dev &&
+                      p->gkey.pax_dev == io->pax_dev && p->gkey.stats_dev == io->stats_dev &&
+                      p->gkey.max_rows == io->max_rows && p->gkey.field_out_dev == io->field_out_dev &&
+                      p->gkey_dz == io->dz;
+    if (!same) {
+      if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
+      if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
+      if (!p->cap_stream) HIP_TRY(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
+      HIP_TRY(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal));
+      enqueue_step(p, io, p->cap_stream);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(p->cap_stream, &g);
+      if (e != hipSuccess) return fail(ASH_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+      p->graph = g;
+      HIP_TRY(hipGraphInstantiate(&p->gexec, p->graph, nullptr, nullptr, 0));
+      p->gkey = *io;
+      p->gkey_dz = io->dz;
+    }
+    HIP_TRY(hipGraphLaunch(p->gexec, s));
+  } else {
+    enqueue_step(p, io, s);
+  }
+  p->host_row = io->row + 1;
+  return check_launch();
+}
+
+int ash_march_step(ash_plan* p, const float* in, float* out, void* stream) {
+  if (!p || !in || !out) return fail(ASH_E_ARG, "null argument");
+  if (!(p->dz > 0)) return fail(ASH_E_STATE, "dz not set");
+  hipStream_t s = (hipStream_t)stream;
+  load_real(p, in, nullptr, nullptr, s);
+  EpiDesc e = epi_base(p);
+  enqueue_composite(p, p->dz, 0, e, s);
+  store_real(p, out, s);
+  p->loaded = true;
+  return check_launch();
+}
+
+int ash_attenuation_step(ash_plan* p, const float* in, float* out, double scale, void* stream) {
+  if (!p || !in || !out) return fail(ASH_E_ARG, "null argument");
+  if (!(p->dz > 0)) return fail(ASH_E_STATE, "dz not set");
+  hipStream_t s = (hipStream_t)stream;
+  load_real(p, in, nullptr, nullptr, s);
+  xfwd(p, s);
+  ypass(p, make_op(p, false, 0.0, true, p->dz * scale, p->dz), s);
+  xinv(p, 0, epi_base(p), s);
+  store_real(p, out, s);
+  p->loaded = true;
+  return check_launch();
+}
+
+int ash_angular_spectrum_step(ash_plan* p, const float* in, float* out, int32_t half, void* stream) {
+  if (!p || !in || !out) return fail(ASH_E_ARG, "null argument");
+  if (!(p->dz > 0)) return fail(ASH_E_STATE, "dz not set");
+  hipStream_t s = (hipStream_t)stream;
+  load_real(p, in, nullptr, nullptr, s);
+  xfwd(p, s);
+  ypass(p, make_op(p, true, half ? p->dz / 2 : p->dz, false, 0.0, p->dz), s);
+  EpiDesc e = epi_base(p);
+  e.half = half ? 1 : 0;
+  xinv(p, 1, e, s);
+  ColDesc cd = col_base(p);
+  cd.mode = COL_TAPER;
+  cd.taper = half ? 1 : 2;
+  column(p, cd, s);
+  store_real(p, out, s);
+  p->loaded = true;
+  return check_launch();
+}
+
+int ash_nonlinear_step(ash_plan* p, const float* in, float* out, int32_t scheme, double dz, double beta,
+                       int32_t subcycle, void* stream) {
+  if (!p || !in || !out) return fail(ASH_E_ARG, "null argument");
+  if (scheme < ASH_NL_NONE || scheme > ASH_NL_KT_MINMOD) return fail(ASH_E_ARG, "unknown scheme");
+  ColDesc cd = col_base(p);
+  cd.mode = COL_RAWNL;
+  cd.nl = scheme;
+  cd.dz = dz;
+  cd.beta_tvd = (float)beta;
+  cd.subcycle = subcycle;
+  cd.rin = in;
+  cd.rout = out;
+  column(p, cd, (hipStream_t)stream);
+  return check_launch();
+}
+
+int ash_max_abs(ash_plan* p, const float* in, uint32_t* out_bits, void* stream) {
+  if (!p || !in || !out_bits) return fail(ASH_E_ARG, "null argument");
+  // the load kernel computes max|p| of its input; the spectrum it writes is
+  // the plan state, so this call also loads `in` as the current field
+  load_real(p, in, out_bits, nullptr, (hipStream_t)stream);
+  p->loaded = true;
+  return check_launch();
+}
+
+int ash_freq_weighted_boundary_step(ash_plan* p, const float* in, float* out, void* stream) {
+  if (!p || !in || !out) return fail(ASH_E_ARG, "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  load_real(p, in, nullptr, nullptr, s);
+  xfwd(p, s);
+  EpiDesc e = epi_base(p);
+  e.fw = 1;
+  e.scale = (float)(1.0 / p->d.nX);
+  xinv(p, 2, e, s);
+  store_real(p, out, s);
+  p->loaded = true;
+  return check_launch();
+}
+
+int ash_super_absorbing_step(ash_plan* p, const float* in, float* out, void* stream) {
+  if (!p || !in || !out) return fail(ASH_E_ARG, "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  load_real(p, in, nullptr, nullptr, s);
+  xfwd(p, s);
+  EpiDesc e = epi_base(p);
+  e.sa = 1;
+  e.scale = (float)(1.0 / p->d.nX);
+  xinv(p, 2, e, s);
+  ScreenDesc sd{};
+  sd.sa = 1;
+  sd.sa_strength = (float)p->d.superAbsorbingStrength;
+  sd.inv2dy = (float)(1.0 / (2 * p->d.dY));
+  sd.invdy = (float)(1.0 / p->d.dY);
+  sd.f0b = 1.f;
+  ysa(p, sd, s);
+  store_real(p, out, s);
+  p->loaded = true;
+  return check_launch();
+}
+
+int ash_apply_phase_screen(ash_plan* p, const float* in, float* out, const float* phase, const float* amp,
+                           void* stream) {
+  if (!p || !in || !out || !phase) return fail(ASH_E_ARG, "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  load_real(p, in, nullptr, nullptr, s);
+  ScreenDesc sd{};
+  sd.sa = 0;
+  sd.phase = phase;
+  sd.amp = amp;
+  const double f0_bin = p->d.f0 / (1.0 / (p->d.nT * p->d.dT));
+  sd.f0b = (float)std::fmax(f0_bin, 1.0);
+  ysa(p, sd, s);
+  store_real(p, out, s);
+  p->loaded = true;
+  return check_launch();
+}
+
+int64_t ash_plan_workspace_bytes(const ash_plan* p) { return p ? p->ws_bytes : 0; }
+
+}  // extern "C"

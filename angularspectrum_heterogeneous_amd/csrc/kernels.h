@@ -1,0 +1,805 @@
+// gfx950 kernels of one z-step of the nonlinear angular-spectrum march.
+//
+// State between kernels: the rfft_t half spectrum S[x][y][f] (complex64),
+// f fastest, row length MP (>= M+1, multiple of 16), M = nT/2.  Bin M is the
+// Nyquist bin, bins > M are padding and never written.
+//
+// Line sweeps (x and y) work on tiles of one line family: all positions along
+// the transform axis for LF=16 consecutive f bins (128-byte segments).  The
+// t-column kernel works on CPB whole columns per block (one wave per column).
+#pragma once
+#include "fft_engine.h"
+
+namespace ash {
+
+constexpr int LF = 16;    // f bins per line-sweep tile
+constexpr int LNT = 256;  // threads per line-sweep block
+constexpr int CPB = 4;    // columns per column-kernel block
+constexpr int TPC = 64;   // threads per column (one wave)
+constexpr int CNT = CPB * TPC;
+
+// ---------------------------------------------------------------------------
+// Generic multi-line Stockham transform.  Layout policy:
+//   LINE_FAST=true : LDS index pos*NL + line, butterfly b -> (line=b%NL, j=b/NL)
+//   LINE_FAST=false: LDS index line*LS + pos, butterfly b -> (line=b/NB, j=b%NB)
+// load(line,pos) feeds pass 0, store(line,pos,v) receives the last pass.
+// ---------------------------------------------------------------------------
+template <int L, int NL, int LS, bool LINE_FAST>
+struct Lay {
+  __device__ static __forceinline__ int idx(int line, int pos) {
+    return LINE_FAST ? pos * NL + line : line * LS + pos;
+  }
+  template <int NB>
+  __device__ static __forceinline__ void map(int b, int& line, int& j) {
+    if (LINE_FAST) { line = b % NL; j = b / NL; }
+    else { line = b / NB; j = b % NB; }
+  }
+};
+
+template <int L, int P, bool INV, int NT, int NL, class LY, class LoadF, class StoreF>
+__device__ __forceinline__ void fft_pass(float2* buf, const float2* tw, int tid, LoadF& load, StoreF& store) {
+  using PR = PassR<L, P, INV>;
+  constexpr int R = PR::R, NB = PR::NB, Ns = PR::Ns, NP = PR::NP;
+  constexpr int TOT = NB * NL;
+  constexpr int PER = (TOT + NT - 1) / NT;
+  float2 v[PER][R];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int b = tid + q * NT;
+    if (TOT % NT == 0 || b < TOT) {
+      int line, j;
+      LY::template map<NB>(b, line, j);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if constexpr (P == 0) v[q][r] = load(line, j + r * NB);
+        else v[q][r] = buf[LY::idx(line, j + r * NB)];
+      }
+      twiddle<L, R, Ns, INV>(v[q], j, tw);
+      dft<R, INV>(v[q]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int b = tid + q * NT;
+    if (TOT % NT == 0 || b < TOT) {
+      int line, j;
+      LY::template map<NB>(b, line, j);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int pos = out_pos<L, P, INV>(j, r);
+        if constexpr (P == NP - 1) store(line, pos, v[q][r]);
+        else buf[LY::idx(line, pos)] = v[q][r];
+      }
+    }
+  }
+  if constexpr (P != NP - 1) __syncthreads();
+}
+
+template <int L, bool INV, int NT, int NL, class LY, class LoadF, class StoreF>
+__device__ __forceinline__ void fft_run(float2* buf, const float2* tw, int tid, LoadF load, StoreF store) {
+  constexpr int NP = Radix<L>::NP;
+  fft_pass<L, 0, INV, NT, NL, LY>(buf, tw, tid, load, store);
+  if constexpr (NP > 1) fft_pass<L, 1, INV, NT, NL, LY>(buf, tw, tid, load, store);
+  if constexpr (NP > 2) fft_pass<L, 2, INV, NT, NL, LY>(buf, tw, tid, load, store);
+}
+
+template <int L>
+__device__ __forceinline__ void load_tw(float2* tw, const float2* __restrict__ g, int tid, int nt) {
+  for (int i = tid; i < L; i += nt) tw[i] = g[i];
+}
+
+// ---------------------------------------------------------------------------
+// Per-plan constant tables (device memory, built on the host in FP64).
+// ---------------------------------------------------------------------------
+struct Tables {
+  const double* kx;      // [nX] natural-order kx (ASS:403-408 centred grid)
+  const double* ky;      // [nY]
+  const double* kt;      // [MP] ASM t-wavenumber, 0 where the bin is not kept
+  const double* ka;      // [MP] attenuation |k| = 2 pi f_m / c0
+  const double* alpha;   // [MP] alpha(f_m)   (Np/m)
+  const double* astar;   // [MP] alphaStar(f_m)
+  const float* tapx_h;   // [nX] sqrt(taper_x)   (split-step abl_half factor)
+  const float* tapy_h;   // [nY]
+  const float* tapt_h;   // [nT]
+  const float* tapx_f;   // [nX] taper_x        (sequential abl factor)
+  const float* tapy_f;
+  const float* tapt_f;
+  const float* lgx;      // [nX] log2(taper_x) for the frequency-weighted mask
+  const float* lgy;      // [nY]
+  const float* fw_exp;   // [MP] exponent clip(f0/max(f_m,f_1),0.1,10)
+  const float* sa_wx;    // [nX] super-absorbing weights
+  const float* sa_wy;    // [nY]
+  const float* sa_cw;    // [MP] c0/omega_m (0 where the coefficient is zeroed)
+  const float2* twx;     // [nX] exp(-2 pi i k / nX)
+  const float2* twy;     // [nY]
+  const float2* twm;     // [M]  exp(-2 pi i k / M)
+  const float2* twn;     // [M+1] exp(-2 pi i k / nT)
+};
+
+struct Geom {
+  int nX, nY, nT, M, MP, nfb;
+};
+
+// Operator descriptor for one k-space visit.
+struct OpDesc {
+  int use_prop;       // angular-spectrum propagator with step dz_prop
+  int use_att;        // attenuation/dispersion with step dz_att
+  int obliquity;      // k/k_z path-length factor
+  int clip_negative;  // water branch: zero the filter where Re < 0
+  double dz_prop;
+  double dz_att;
+  double k_start, k_max;  // transverse cosine filter band
+  float norm;             // 1/(nX nY) of the inverse 2-D FFT
+};
+
+// Propagator x attenuation at natural bin (i, j, m); kk = kx^2 + ky^2 (FP64).
+__device__ __forceinline__ float2 visit_operator(const OpDesc& op, double kk, double kt, double ka,
+                                                 double al, double as, int m) {
+  float2 out = make_float2(op.norm, 0.f);
+  if (op.use_prop) {
+    if (kt <= 0.0) return make_float2(0.f, 0.f);  // DC, Nyquist, negative and padding bins
+    const double ktr = sqrt(kk);
+    float filt = 1.f;
+    if (ktr > op.k_max) return make_float2(0.f, 0.f);
+    if (ktr > op.k_start) {
+      const float pos = (float)((ktr - op.k_start) / (op.k_max - op.k_start));
+      filt = 0.5f * (1.f + cospif(pos));
+    }
+    const double d2 = kt * kt - kk;  // FP64: cancels near the evanescent cutoff
+    float ph, amp = filt;
+    if (d2 > 0.0) {
+      const float kz = sqrtf((float)d2);
+      ph = (float)op.dz_prop * ((float)kk / ((float)kt + kz));  // dz (kt - kz)
+    } else {
+      ph = (float)(op.dz_prop * kt);
+      amp *= expf(-(float)op.dz_prop * sqrtf((float)(-d2)));
+    }
+    float s, c;
+    sincosf(ph, &s, &c);
+    out = make_float2(out.x * amp * c, out.x * amp * s);
+  }
+  if (op.use_att && m > 0) {
+    float re, im;
+    bool apply = true;
+    if (op.obliquity) {
+      const double kz2 = ka * ka - kk;
+      if (kz2 > 0.0) {
+        const double kz = fmax(sqrt(kz2), ka * 1e-6);
+        const double obl = ka / kz;
+        re = (float)(-al * op.dz_att * obl);
+        im = (float)(-as * op.dz_att * obl);
+      } else {
+        apply = false;
+      }
+    } else {
+      re = (float)(-al * op.dz_att);
+      im = (float)(-as * op.dz_att);
+    }
+    if (apply) {
+      const float a = expf(re);
+      float s, c;
+      sincosf(im, &s, &c);
+      float2 fac = make_float2(a * c, a * s);
+      if (op.clip_negative && fac.x < 0.f) fac = make_float2(0.f, 0.f);
+      out = cmul(out, fac);
+    }
+  }
+  return out;
+}
+
+// ---------------------------------------------------------------------------
+// x-sweep, forward FFT along x.  Tile = (y, f-block).
+// ---------------------------------------------------------------------------
+template <int L>
+__global__ __launch_bounds__(LNT) void k_xfft_fwd(float2* __restrict__ spec, Tables T, Geom g) {
+  __shared__ float2 tw[L];
+  __shared__ float2 buf[L * LF];
+  const int tid = threadIdx.x;
+  const int y = blockIdx.x / g.nfb, fb = blockIdx.x % g.nfb;
+  const int f0 = fb * LF;
+  const long sx = (long)g.nY * g.MP;
+  float2* base = spec + (long)y * g.MP + f0;
+  const int nvalid = min(LF, g.M + 1 - f0);
+  load_tw<L>(tw, T.twx, tid, LNT);
+  __syncthreads();
+  using LY = Lay<L, LF, 0, true>;
+  auto ld = [&](int line, int pos) -> float2 {
+    return line < nvalid ? base[pos * sx + line] : make_float2(0.f, 0.f);
+  };
+  auto st = [&](int line, int pos, float2 v) {
+    if (line < nvalid) base[pos * sx + line] = v;
+  };
+  fft_run<L, false, LNT, LF, LY>(buf, tw, tid, ld, st);
+}
+
+// ---------------------------------------------------------------------------
+// y-pass: forward FFT along y, multiply by the visit operator, inverse FFT.
+// Tile = (kx, f-block).  The 1/(nX nY) normalisation is in the operator.
+// ---------------------------------------------------------------------------
+template <int L>
+__global__ __launch_bounds__(LNT) void k_ypass(float2* __restrict__ spec, Tables T, Geom g, OpDesc op) {
+  __shared__ float2 tw[L];
+  __shared__ float2 buf[L * LF];
+  __shared__ double sky[L];
+  __shared__ double skt[LF], ska[LF], sal[LF], sas[LF];
+  const int tid = threadIdx.x;
+  const int x = blockIdx.x / g.nfb, fb = blockIdx.x % g.nfb;
+  const int f0 = fb * LF;
+  float2* base = spec + (long)x * g.nY * g.MP + f0;
+  const long sy = g.MP;
+  const int nvalid = min(LF, g.M + 1 - f0);
+  load_tw<L>(tw, T.twy, tid, LNT);
+  for (int i = tid; i < L; i += LNT) sky[i] = T.ky[i];
+  if (tid < LF) {
+    const int m = min(f0 + tid, g.MP - 1);
+    skt[tid] = T.kt[m]; ska[tid] = T.ka[m]; sal[tid] = T.alpha[m]; sas[tid] = T.astar[m];
+  }
+  __syncthreads();
+  const double kx = T.kx[x];
+  const double kx2 = kx * kx;
+  using LY = Lay<L, LF, 0, true>;
+  auto ld = [&](int line, int pos) -> float2 {
+    return line < nvalid ? base[pos * sy + line] : make_float2(0.f, 0.f);
+  };
+  auto st_lds = [&](int line, int pos, float2 v) { buf[LY::idx(line, pos)] = v; };
+  fft_run<L, false, LNT, LF, LY>(buf, tw, tid, ld, st_lds);
+  // the inverse pass 0 reads back from LDS through the operator
+  auto ld_op = [&](int line, int pos) -> float2 {
+    const float2 v = buf[LY::idx(line, pos)];
+    const double kk = kx2 + sky[pos] * sky[pos];
+    const float2 o = visit_operator(op, kk, skt[line], ska[line], sal[line], sas[line], f0 + line);
+    return cmul(v, o);
+  };
+  auto st = [&](int line, int pos, float2 v) {
+    if (line < nvalid) base[pos * sy + line] = v;
+  };
+  __syncthreads();
+  fft_run<L, true, LNT, LF, LY>(buf, tw, tid, ld_op, st);
+}
+
+// ---------------------------------------------------------------------------
+// x-sweep, inverse FFT along x + epilogue.
+//   EPI 0: none
+//   EPI 1: multiply by tapx[x] tapy[y] (abl factor in x,y)
+//   EPI 2: project DC/Nyquist to real, frequency-weighted mask, super-absorbing x pass
+// ---------------------------------------------------------------------------
+struct EpiDesc {
+  int half;           // EPI 1: use the sqrt tapers (split-step) instead of the full ones
+  int fw;             // EPI 2: frequency-weighted boundary on
+  int sa;             // EPI 2: super-absorbing x pass on
+  float sa_strength;
+  float inv2dx, invdx;  // 1/(2 dX), 1/dX
+  float scale;          // EPI 2: factor on the transformed values (1/nX when no y-pass normalised them)
+};
+
+template <int L, int EPI>
+__global__ __launch_bounds__(LNT) void k_xfft_inv(float2* __restrict__ spec, Tables T, Geom g, EpiDesc e) {
+  __shared__ float2 tw[L];
+  __shared__ float2 buf[L * LF];
+  const int tid = threadIdx.x;
+  const int y = blockIdx.x / g.nfb, fb = blockIdx.x % g.nfb;
+  const int f0 = fb * LF;
+  const long sx = (long)g.nY * g.MP;
+  float2* base = spec + (long)y * g.MP + f0;
+  const int nvalid = min(LF, g.M + 1 - f0);
+  load_tw<L>(tw, T.twx, tid, LNT);
+  __syncthreads();
+  using LY = Lay<L, LF, 0, true>;
+  auto ld = [&](int line, int pos) -> float2 {
+    return line < nvalid ? base[pos * sx + line] : make_float2(0.f, 0.f);
+  };
+  if constexpr (EPI == 0 || EPI == 1) {
+    float ty = 1.f;
+    const float* tx = nullptr;
+    if constexpr (EPI == 1) {
+      ty = e.half ? T.tapy_h[y] : T.tapy_f[y];
+      tx = e.half ? T.tapx_h : T.tapx_f;
+    }
+    auto st = [&](int line, int pos, float2 v) {
+      if constexpr (EPI == 1) v = cscale(v, tx[pos] * ty);
+      if (line < nvalid) base[pos * sx + line] = v;
+    };
+    fft_run<L, true, LNT, LF, LY>(buf, tw, tid, ld, st);
+  } else {
+    auto st_lds = [&](int line, int pos, float2 v) { buf[LY::idx(line, pos)] = v; };
+    fft_run<L, true, LNT, LF, LY>(buf, tw, tid, ld, st_lds);
+    __syncthreads();
+    const float lgy = T.lgy[y];
+    // element e -> (line, pos), line fastest: coalesced 128-byte rows
+    constexpr int PER = (L * LF) / LNT;
+    float2 res[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int el = tid + q * LNT;
+      const int line = el % LF, pos = el / LF;
+      const int m = f0 + line;
+      float2 v = cscale(buf[LY::idx(line, pos)], e.scale);
+      if (m == 0 || m == g.M) v.y = 0.f;  // irfft/rfft round trip of the reference
+      float fwm = 1.f;
+      if (e.fw) fwm = exp2f(T.fw_exp[min(m, g.MP - 1)] * (T.lgx[pos] + lgy));
+      v = cscale(v, fwm);
+      if (e.sa) {
+        const float w = T.sa_wx[pos];
+        if (w != 0.f) {
+          // neighbours, with the same projection and mask applied
+          auto nb = [&](int p) -> float2 {
+            float2 u = cscale(buf[LY::idx(line, p)], e.scale);
+            if (m == 0 || m == g.M) u.y = 0.f;
+            float f = 1.f;
+            if (e.fw) f = exp2f(T.fw_exp[min(m, g.MP - 1)] * (T.lgx[p] + lgy));
+            return cscale(u, f);
+          };
+          float2 d;
+          if (pos == 0) d = cscale(csub(nb(1), v), e.invdx);
+          else if (pos == L - 1) d = cscale(csub(v, nb(L - 2)), e.invdx);
+          else d = cscale(csub(nb(pos + 1), nb(pos - 1)), e.inv2dx);
+          const float cw = T.sa_cw[min(m, g.MP - 1)];
+          // coeff * dF with coeff = -i c0/omega
+          const float2 cd = make_float2(cw * d.y, -cw * d.x);
+          const float2 pin = pos < L / 2 ? cscale(cadd(v, cd), 0.5f) : cscale(csub(v, cd), 0.5f);
+          v = csub(v, cscale(pin, e.sa_strength * w));
+        }
+      }
+      res[q] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int el = tid + q * LNT;
+      const int line = el % LF, pos = el / LF;
+      if (line < nvalid) base[pos * sx + line] = res[q];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// y-sweep without FFT: super-absorbing y pass, DC/Nyquist projection and an
+// optional phase/amplitude screen.  Tile = (x, f-block).
+// ---------------------------------------------------------------------------
+struct ScreenDesc {
+  const float* phase;  // [nX*nY] or null
+  const float* amp;    // [nX*nY] or null
+  float f0b;           // max(f0_bin, 1)
+  int sa;
+  float sa_strength;
+  float inv2dy, invdy;
+};
+
+template <int L>
+__global__ __launch_bounds__(LNT) void k_ysa(float2* __restrict__ spec, Tables T, Geom g, ScreenDesc s) {
+  __shared__ float2 buf[L * LF];
+  const int tid = threadIdx.x;
+  const int x = blockIdx.x / g.nfb, fb = blockIdx.x % g.nfb;
+  const int f0 = fb * LF;
+  float2* base = spec + (long)x * g.nY * g.MP + f0;
+  const long sy = g.MP;
+  const int nvalid = min(LF, g.M + 1 - f0);
+  constexpr int PER = (L * LF) / LNT;
+  float2 v[PER];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int el = tid + q * LNT;
+    const int line = el % LF, pos = el / LF;
+    v[q] = line < nvalid ? base[pos * sy + line] : make_float2(0.f, 0.f);
+    buf[pos * LF + line] = v[q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int el = tid + q * LNT;
+    const int line = el % LF, pos = el / LF;
+    const int m = f0 + line;
+    float2 u = v[q];
+    if (s.sa) {
+      const float w = T.sa_wy[pos];
+      if (w != 0.f) {
+        float2 d;
+        if (pos == 0) d = cscale(csub(buf[1 * LF + line], u), s.invdy);
+        else if (pos == L - 1) d = cscale(csub(u, buf[(L - 2) * LF + line]), s.invdy);
+        else d = cscale(csub(buf[(pos + 1) * LF + line], buf[(pos - 1) * LF + line]), s.inv2dy);
+        const float cw = T.sa_cw[min(m, g.MP - 1)];
+        const float2 cd = make_float2(cw * d.y, -cw * d.x);
+        const float2 pin = pos < L / 2 ? cscale(cadd(u, cd), 0.5f) : cscale(csub(u, cd), 0.5f);
+        u = csub(u, cscale(pin, s.sa_strength * w));
+      }
+    }
+    if (m == 0 || m == g.M) u.y = 0.f;
+    if (s.phase) {
+      const long idx = (long)x * g.nY + pos;
+      const float fs = (float)m / s.f0b;
+      const float th = s.phase[idx] * fs;
+      float sn, cs;
+      sincosf(th, &sn, &cs);
+      u = cmul(u, make_float2(cs, -sn));
+      if (s.amp) {
+        const float a = fminf(fmaxf(s.amp[idx], 1e-6f), 1.f);
+        u = cscale(u, powf(a, fs));
+      }
+      if (m == 0 || m == g.M) u.y = 0.f;
+    }
+    v[q] = u;
+  }
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int el = tid + q * LNT;
+    const int line = el % LF, pos = el / LF;
+    if (line < nvalid) base[pos * sy + line] = v[q];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// t-column kernel: irfft -> real-space column work -> rfft.
+// ---------------------------------------------------------------------------
+enum ColMode : int {
+  COL_LOAD = 0,    // real field in -> spectrum (+ I_before, max|p| of the input)
+  COL_NL = 1,      // taper * nonlinear step
+  COL_TAPER = 2,   // taper only
+  COL_FINAL = 3,   // sources + reductions (+ spectrum rewrite if a source was added)
+  COL_STORE = 4,   // spectrum -> real field
+  COL_RAWNL = 5,   // real field in -> nonlinear step -> real field out (no FFT)
+};
+
+enum NLKind : int { NL_NONE = 0, NL_KT = 1, NL_RUSANOV = 2, NL_TVD = 3, NL_KT_MINMOD = 4 };
+
+struct ColDesc {
+  int mode;
+  int nl;            // NLKind
+  int taper;         // 0 none, 1 sqrt taper (split), 2 full taper (sequential)
+  int subcycle;      // 0 off, 1 per column
+  float cfl_target;
+  double N, dT, dz;  // Burgers coefficient, time step, z step of the nonlinear update
+  float beta_tvd;
+  // real-field io
+  const float* rin;  // COL_LOAD / COL_RAWNL input [nX*nY*nT]
+  float* rout;       // COL_STORE / COL_RAWNL / COL_FINAL(optional) output
+  // reductions (COL_FINAL / COL_LOAD)
+  const int* row;    // device step counter; outputs go to row `*row`
+  int max_rows;
+  float* pnp; float* ppp; float* pI; float* pIloss; float* pax;
+  unsigned* stats;   // [max_rows+1] max|p| bits; COL_LOAD writes slot 0, COL_FINAL slot row+1
+  double* ibefore;   // [nX*nY] previous-step intensity (FP64)
+  int cax;           // index of the on-axis column (nX/2, nY/2)
+  // sources (COL_FINAL)
+  const float* src[4];
+  int n_src;
+  const unsigned char* smask;
+  const double* sdelay;
+  const double* taxis;
+  double s_f0, s_p0, s_ncyc, s_dur;
+};
+
+// MC / minmod slope of cell i from its neighbours (ASS:727-755); float32 literal ops
+__device__ __forceinline__ float sgnf(float a) { return a > 0.f ? 1.f : (a < 0.f ? -1.f : 0.f); }
+__device__ __forceinline__ float minmod2f(float a, float b) {
+  return (a * b > 0.f) ? sgnf(a) * fminf(fabsf(a), fabsf(b)) : 0.f;
+}
+__device__ __forceinline__ float minmod3f(float a, float b, float c) {
+  const bool same = (a * b > 0.f) && (b * c > 0.f);
+  const float m = fminf(fminf(fabsf(a), fabsf(b)), fabsf(c));
+  return same ? sgnf(a) * m : 0.f;
+}
+template <bool MC>
+__device__ __forceinline__ float slope(float um, float u0, float up) {
+  const float dm = u0 - um, dp = up - u0;
+  if (MC) return minmod3f(2.f * dm, 0.5f * (dm + dp), 2.f * dp);
+  return minmod2f(dm, dp);
+}
+// KT central-upwind flux at an interface (ASS:771-793)
+__device__ __forceinline__ float kt_iface(float umi, float upi) {
+  const float ap = fmaxf(fmaxf(-umi, -upi), 0.f);
+  const float am = fminf(fminf(-umi, -upi), 0.f);
+  const float fm = -0.5f * umi * umi;
+  const float fp = -0.5f * upi * upi;
+  const float den = ap - am;
+  if (fabsf(den) < 1e-14f) {
+    return 0.5f * (fm + fp) - 0.5f * fmaxf(fabsf(umi), fabsf(upi)) * (upi - umi);
+  }
+  return (ap * fm - am * fp - ap * am * (upi - umi)) / den;
+}
+
+// Semi-discrete KT rhs for S samples t0..t0+S-1 from u[-2..S+1] (index offset 2).
+template <int S, bool MC>
+__device__ __forceinline__ void kt_rhs_local(const float* u, int t0, int nT, float coef, float* rhs) {
+  float sig[S + 2];
+#pragma unroll
+  for (int i = 0; i < S + 2; ++i) {
+    const int t = t0 - 1 + i;
+    sig[i] = (t >= 1 && t <= nT - 2) ? slope<MC>(u[i], u[i + 1], u[i + 2]) : 0.f;
+  }
+  float fl[S + 1];
+#pragma unroll
+  for (int i = 0; i < S + 1; ++i) {
+    // interface between t = t0-1+i and t0+i ; valid for 0 <= t0-1+i <= nT-2
+    const float um = u[i + 1] + 0.5f * sig[i];
+    const float up = u[i + 2] - 0.5f * sig[i + 1];
+    fl[i] = kt_iface(um, up);
+  }
+#pragma unroll
+  for (int i = 0; i < S; ++i) {
+    const int t = t0 + i;
+    rhs[i] = (t >= 1 && t <= nT - 2) ? -coef * (fl[i + 1] - fl[i]) : 0.f;
+  }
+}
+
+template <int M>
+__global__ __launch_bounds__(CNT) void k_column(float2* __restrict__ spec, Tables T, Geom g, ColDesc d) {
+  constexpr int NT_ = 2 * M;
+  constexpr int LS = M + 1;
+  constexpr int S = NT_ / TPC;  // real samples per thread
+  __shared__ float2 twm[M];
+  __shared__ float2 twn[M + 1];
+  __shared__ float2 buf[CPB * LS];
+  __shared__ float redmax[CPB];
+  const int tid = threadIdx.x;
+  const int col = tid / TPC, lane = tid % TPC;
+  const int ncols = g.nX * g.nY;
+  const long c = (long)blockIdx.x * CPB + col;
+  const bool live = c < ncols;
+  for (int i = tid; i < M; i += CNT) twm[i] = T.twm[i];
+  for (int i = tid; i <= M; i += CNT) twn[i] = T.twn[i];
+  float2* cbuf = buf + col * LS;
+  float* rb = reinterpret_cast<float*>(cbuf);  // the real column aliases the spectrum buffer
+  float2* sc = spec + c * g.MP;
+  using LY = Lay<M, CPB, LS, false>;
+  const float invN = 1.f / (float)NT_;
+
+  // ---- stage 1: obtain the real column in rb[0..nT) ----
+  if (d.mode == COL_LOAD || d.mode == COL_RAWNL) {
+    const float* src = d.rin + c * NT_;
+    for (int i = lane; i < NT_; i += TPC) rb[i] = live ? src[i] : 0.f;
+    __syncthreads();
+  } else {
+    for (int k = lane; k <= M; k += TPC) {
+      float2 v = live ? sc[k] : make_float2(0.f, 0.f);
+      if (k == 0 || k == M) v.y = 0.f;
+      cbuf[k] = v;
+    }
+    __syncthreads();
+    // c2r: Z[k] = (X_k + conj X_{M-k}) + i W_N^{-k} (X_k - conj X_{M-k}), then inverse M-point FFT
+    auto ldz = [&](int line, int k) -> float2 {
+      const float2* cb = buf + line * LS;
+      const float2 a = cb[k], b = cconj(cb[M - k]);
+      const float2 s = cadd(a, b), df = csub(a, b);
+      const float2 w = cconj(twn[k]);
+      const float2 t = cmul(w, df);
+      return make_float2((s.x - t.y) * 0.5f, (s.y + t.x) * 0.5f);
+    };
+    auto str = [&](int line, int n, float2 v) { buf[line * LS + n] = cscale(v, 2.f * invN); };
+    fft_run<M, true, CNT, CPB, LY>(buf, twm, tid, ldz, str);
+    __syncthreads();
+  }
+
+  // ---- stage 2: real-space work on S contiguous samples per lane ----
+  const int t0 = lane * S;
+  float u[S];
+#pragma unroll
+  for (int i = 0; i < S; ++i) u[i] = rb[t0 + i];
+  bool rewrite = true;  // whether the spectrum is written back
+
+  if (d.mode == COL_NL || d.mode == COL_TAPER || d.mode == COL_RAWNL) {
+    if (d.taper) {
+      const float* tt = d.taper == 1 ? T.tapt_h : T.tapt_f;
+#pragma unroll
+      for (int i = 0; i < S; ++i) u[i] *= tt[t0 + i];
+    }
+    if (d.nl != NL_NONE && (d.mode == COL_NL || d.mode == COL_RAWNL)) {
+      int nsub = 1;
+      float dz = (float)d.dz;
+      if (d.subcycle) {
+        float mx = 0.f;
+#pragma unroll
+        for (int i = 0; i < S; ++i) mx = fmaxf(mx, fabsf(u[i]));
+        for (int o = TPC / 2; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, TPC));
+        const double cfl = d.N * (double)mx * d.dz / d.dT;
+        if (!(cfl <= (double)d.cfl_target || mx < 1e-30f)) nsub = (int)ceil(cfl / (double)d.cfl_target);
+        dz = (float)(d.dz / nsub);
+      }
+      // barriers inside the loop need a block-uniform trip count: run the
+      // block maximum and let columns that are done idle through the rest
+      __shared__ int snsub[CPB];
+      if (lane == 0) snsub[col] = nsub;
+      __syncthreads();
+      int nsub_blk = snsub[0];
+      for (int k = 1; k < CPB; ++k) nsub_blk = max(nsub_blk, snsub[k]);
+      const float coef = (float)(d.N / d.dT);
+      for (int sub = 0; sub < nsub_blk; ++sub) {
+        const bool act = sub < nsub;
+        if (d.nl == NL_KT || d.nl == NL_KT_MINMOD) {
+          // SSP-RK2 (ASS:833-839)
+          float h[S + 4], r0[S], s1[S];
+          __syncthreads();
+#pragma unroll
+          for (int i = 0; i < S; ++i) rb[t0 + i] = u[i];
+          __syncthreads();
+#pragma unroll
+          for (int i = 0; i < S + 4; ++i) {
+            const int t = t0 - 2 + i;
+            h[i] = (t >= 0 && t < NT_) ? rb[t] : 0.f;
+          }
+          if (d.nl == NL_KT) kt_rhs_local<S, true>(h, t0, NT_, coef, r0);
+          else kt_rhs_local<S, false>(h, t0, NT_, coef, r0);
+#pragma unroll
+          for (int i = 0; i < S; ++i) s1[i] = act ? u[i] + dz * r0[i] : u[i];
+          __syncthreads();
+#pragma unroll
+          for (int i = 0; i < S; ++i) rb[t0 + i] = s1[i];
+          __syncthreads();
+#pragma unroll
+          for (int i = 0; i < S + 4; ++i) {
+            const int t = t0 - 2 + i;
+            h[i] = (t >= 0 && t < NT_) ? rb[t] : 0.f;
+          }
+          if (d.nl == NL_KT) kt_rhs_local<S, true>(h, t0, NT_, coef, r0);
+          else kt_rhs_local<S, false>(h, t0, NT_, coef, r0);
+          if (act) {
+#pragma unroll
+            for (int i = 0; i < S; ++i) u[i] = 0.5f * u[i] + 0.5f * (s1[i] + dz * r0[i]);
+          }
+        } else {
+          // forward-Euler Rusanov (ASS:688-724)
+          float h[S + 2];
+          __syncthreads();
+#pragma unroll
+          for (int i = 0; i < S; ++i) rb[t0 + i] = u[i];
+          __syncthreads();
+#pragma unroll
+          for (int i = 0; i < S + 2; ++i) {
+            const int t = t0 - 1 + i;
+            h[i] = (t >= 0 && t < NT_) ? rb[t] : 0.f;
+          }
+          const float cf = (float)(d.N * (double)dz / d.dT);
+          float fl[S + 1];
+#pragma unroll
+          for (int i = 0; i < S + 1; ++i) {
+            const float uL = h[i], uR = h[i + 1];
+            const float a = fmaxf(fabsf(uL), fabsf(uR));
+            if (d.nl == NL_RUSANOV) fl[i] = -(uL * uL + uR * uR) / 2.f - a * (uR - uL);
+            else fl[i] = 0.5f * (-0.5f * (uL * uL + uR * uR)) - 0.5f * a * (uR - uL);
+          }
+#pragma unroll
+          for (int i = 0; i < S; ++i) {
+            const int t = t0 + i;
+            if (act && t >= 1 && t <= NT_ - 2) {
+              const float fd = fl[i + 1] - fl[i];
+              if (d.nl == NL_RUSANOV) {
+                u[i] += -cf * fd;
+              } else {
+                const float dm = h[i + 1] - h[i], dp = h[i + 2] - h[i + 1];
+                const float eps = 1e-30f, b = d.beta_tvd;
+                const float r = dp / (dm + eps * sgnf(dm + eps));
+                const float ri = dm / (dp + eps * sgnf(dp + eps));
+                const float pp = fmaxf(0.f, fminf(fminf(b * r, 1.f), fminf(r, b)));
+                const float pm = fmaxf(0.f, fminf(fminf(b * ri, 1.f), fminf(ri, b)));
+                u[i] += -cf * (0.5f * (pp + pm) * fd);
+              }
+            }
+          }
+        }
+      }
+    }
+  }
+
+  if (d.mode == COL_FINAL) {
+    rewrite = false;
+    const int row = *d.row;
+    if (d.n_src > 0 || d.smask) {
+      rewrite = true;
+      for (int k = 0; k < d.n_src; ++k) {
+        const float* sp = d.src[k] + c * NT_;
+#pragma unroll
+        for (int i = 0; i < S; ++i) u[i] += live ? sp[t0 + i] : 0.f;
+      }
+      if (d.smask && live && d.smask[c]) {
+        const double dl = d.sdelay[c];
+        const double w0 = 2.0 * 3.14159265358979323846 * d.s_f0;
+        const double cexp = 2.0 * d.s_dur;
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+          const double tm = d.taxis[t0 + i] - dl;
+          const double env = exp(-pow(1.05 * tm * w0 / (d.s_ncyc * 3.14159265358979323846), cexp));
+          u[i] += (float)(env * sin(w0 * tm) * d.s_p0);
+        }
+      }
+    }
+    // reductions over t (ASS:1529-1535); intensity in FP64
+    float mn = u[0], mx = u[0];
+    double e = 0.0;
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+      mn = fminf(mn, u[i]);
+      mx = fmaxf(mx, u[i]);
+      e += (double)u[i] * (double)u[i];
+    }
+    for (int o = TPC / 2; o > 0; o >>= 1) {
+      mn = fminf(mn, __shfl_xor(mn, o, TPC));
+      mx = fmaxf(mx, __shfl_xor(mx, o, TPC));
+      e += __shfl_xor(e, o, TPC);
+    }
+    if (live && row < d.max_rows) {
+      const long o = (long)row * ncols + c;
+      if (lane == 0) {
+        const double ib = d.ibefore[c];
+        d.pnp[o] = mn;
+        d.ppp[o] = mx;
+        d.pI[o] = (float)e;
+        d.pIloss[o] = (float)fmax(0.0, ib - e);
+        d.ibefore[c] = e;
+      }
+      if (c == d.cax) {
+#pragma unroll
+        for (int i = 0; i < S; ++i) d.pax[(long)row * NT_ + t0 + i] = u[i];
+      }
+    }
+    const float am = live ? fmaxf(mx, -mn) : 0.f;
+    if (lane == 0) redmax[col] = am;
+    __syncthreads();
+    if (tid == 0) {
+      float m = redmax[0];
+      for (int k = 1; k < CPB; ++k) m = fmaxf(m, redmax[k]);
+      if (row < d.max_rows) atomicMax(d.stats + row + 1, __float_as_uint(m));
+    }
+    if (d.rout && live) {
+#pragma unroll
+      for (int i = 0; i < S; ++i) d.rout[c * NT_ + t0 + i] = u[i];
+    }
+  } else if (d.mode == COL_LOAD) {
+    float mx = 0.f;
+    double e = 0.0;
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+      mx = fmaxf(mx, fabsf(u[i]));
+      e += (double)u[i] * (double)u[i];
+    }
+    for (int o = TPC / 2; o > 0; o >>= 1) {
+      mx = fmaxf(mx, __shfl_xor(mx, o, TPC));
+      e += __shfl_xor(e, o, TPC);
+    }
+    if (live && lane == 0 && d.ibefore) d.ibefore[c] = e;
+    if (lane == 0) redmax[col] = live ? mx : 0.f;
+    __syncthreads();
+    if (tid == 0 && d.stats) {
+      float m = redmax[0];
+      for (int k = 1; k < CPB; ++k) m = fmaxf(m, redmax[k]);
+      atomicMax(d.stats, __float_as_uint(m));
+    }
+  } else if (d.mode == COL_STORE) {
+    rewrite = false;
+    if (live) {
+#pragma unroll
+      for (int i = 0; i < S; ++i) d.rout[c * NT_ + t0 + i] = u[i];
+    }
+  } else if (d.mode == COL_RAWNL) {
+    rewrite = false;
+    if (live) {
+#pragma unroll
+      for (int i = 0; i < S; ++i) d.rout[c * NT_ + t0 + i] = u[i];
+    }
+  }
+  // block-uniform: the branch below contains barriers
+  bool any_rewrite = __syncthreads_or(rewrite ? 1 : 0);
+  if (!any_rewrite) return;
+
+  // ---- stage 3: r2c of the real column back into the spectrum ----
+#pragma unroll
+  for (int i = 0; i < S; ++i) rb[t0 + i] = u[i];
+  __syncthreads();
+  auto ldr = [&](int line, int k) -> float2 { return buf[line * LS + k]; };
+  auto stz = [&](int line, int k, float2 v) { buf[line * LS + k] = v; };
+  fft_run<M, false, CNT, CPB, LY>(buf, twm, tid, ldr, stz);
+  __syncthreads();
+  if (rewrite && live) {
+    for (int k = lane; k <= M; k += TPC) {
+      const float2 zk = cbuf[k == M ? 0 : k];
+      const float2 zm = cconj(cbuf[k == 0 ? 0 : M - k]);
+      const float2 s = cadd(zk, zm), df = csub(zk, zm);
+      const float2 t = cmul(twn[k], df);  // W_N^k (Z_k - conj Z_{M-k})
+      // X_k = s/2 - i t/2
+      sc[k] = make_float2(0.5f * (s.x + t.y), 0.5f * (s.y - t.x));
+    }
+  }
+}
+
+__global__ void k_set_row(int* row, int v) { *row = v; }
+__global__ void k_advance_row(int* row) { *row += 1; }
+
+}  // namespace ash

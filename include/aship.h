@@ -1,0 +1,184 @@
+/*
+ * aship: C ABI of the MI355X (gfx950) nonlinear angular-spectrum marching library.
+ *
+ * The library replaces the per-z marching step of the reference solver
+ * pinton-lab/angularspectrum_heterogeneous, module angular_spectrum_solver.py
+ * (cited as ASS:<line>).  The reference has no FFI; its boundary is the Python
+ * API (SolverParams, angular_spectrum_solve) and the jitted step functions.
+ * Each entry point below names the reference function whose behaviour it
+ * reproduces.
+ *
+ * Conventions
+ *   - Every pointer argument named *_dev is device memory owned by the caller
+ *     (PyTorch tensors in the Python host code).  Real fields are float32,
+ *     C-contiguous (nX, nY, nT) with t fastest, exactly the reference layout.
+ *   - `stream` is a hipStream_t passed as void*; NULL means the null stream.
+ *     All work is enqueued asynchronously on it.
+ *   - Return value 0 on success, a negative ASH_E_* code on failure; the text
+ *     of the last error of the calling thread is returned by ash_last_error().
+ *     No exception or abort crosses the ABI.
+ *   - A plan is not thread-safe: calls on one plan must be serialised by the
+ *     caller.  Distinct plans are independent.
+ *   - The plan owns only its workspace (the spectral state of the field, FP64
+ *     intensity carry, operator tables and the captured step graph).
+ */
+#ifndef ASHIP_H
+#define ASHIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ASH_ABI_VERSION 1
+
+#define ASH_OK 0
+#define ASH_E_ARG (-1)         /* invalid argument */
+#define ASH_E_UNSUPPORTED (-2) /* grid size or option not implemented */
+#define ASH_E_HIP (-3)         /* HIP runtime error */
+#define ASH_E_STATE (-4)       /* call out of order (e.g. step before load) */
+
+/* boundaryProfile */
+#define ASH_PROFILE_QUADRATIC 0
+#define ASH_PROFILE_WENDLAND 1
+/* nonlinear scheme (dispatch of ASS:1454-1492) */
+#define ASH_NL_NONE 0
+#define ASH_NL_KT 1          /* _kt_flux, ASS:833-839 */
+#define ASH_NL_RUSANOV 2     /* _rusanov_flux_standard, ASS:688-696 */
+#define ASH_NL_TVD 3         /* _rusanov_flux_tvd, ASS:699-724 */
+#define ASH_NL_KT_MINMOD 4   /* _kt_flux_minmod, ASS:862-868 */
+/* sub-cycling of the KT update (0 = none, as the reference march) */
+#define ASH_SUBCYCLE_OFF 0
+#define ASH_SUBCYCLE_COLUMN 1
+
+/* Solver description: the fields of SolverParams (ASS:35-76) that shape the
+ * march, plus the grid.  Flags are 0/1. */
+typedef struct ash_desc {
+  int32_t nX, nY, nT;
+  double dX, dY, dT;
+  double c0, rho0, beta;
+  double alpha0;          /* dB/MHz^pow/cm; negative selects water (ASS:1316) */
+  double attenPow;
+  double f0;
+  double boundaryFactor;
+  int32_t useSplitStep;
+  int32_t useAdaptiveFiltering;
+  int32_t useTVD;
+  double freqFilterThreshold;
+  double adaptiveFilterStrength;
+  int32_t useBoundaryLayer;
+  int32_t boundaryProfile;         /* ASH_PROFILE_* */
+  int32_t useFreqWeightedBoundary;
+  int32_t useSuperAbsorbing;
+  double superAbsorbingStrength;
+  int32_t fluxScheme;              /* 0 rusanov, 1 kt (ASS:64) */
+  int32_t useObliquityCorrection;
+  int32_t subcycle;                /* ASH_SUBCYCLE_* (extension, default off) */
+  double cflTarget;                /* target CFL of sub-cycling (0.5 in ASS:842) */
+} ash_desc;
+
+typedef struct ash_plan ash_plan;
+
+/* Per-step inputs and outputs of ash_step.  Output arrays are device buffers
+ * laid out (max_rows, nX, nY) (and (max_rows, nT) for pax): row r holds step
+ * r.  The reference stores the same values at [:, :, r] (ASS:1530-1535). */
+typedef struct ash_step_io {
+  int32_t row;                 /* step index cc */
+  int32_t max_rows;
+  float* pnp_dev;
+  float* ppp_dev;
+  float* pI_dev;
+  float* pIloss_dev;
+  float* pax_dev;
+  uint32_t* stats_dev;         /* [max_rows+1] float bits of max|p|; slot r+1 after step r */
+  float* field_out_dev;        /* optional: real field after the step (NULL = not written) */
+  double dz;                   /* this step's dZ (the last step may be shorter) */
+  /* phase/amplitude screens crossed in this step (ASS:1505-1518), in order */
+  int32_t n_screens;
+  const float* screen_phase_dev[4];
+  const float* screen_amp_dev[4];  /* entries may be NULL */
+  /* dense source slices crossed in this step (ASS:1521-1527), in order */
+  int32_t n_src_dense;
+  const float* src_dense_dev[4];
+  /* sparse bowl slice (mask + delay form of make_bowl_source_planes slices) */
+  const uint8_t* src_mask_dev;     /* [nX*nY] or NULL */
+  const double* src_delay_dev;     /* [nX*nY] */
+  const double* taxis_dev;         /* [nT] */
+  double src_f0, src_p0, src_ncycles, src_dur;
+} ash_step_io;
+
+/* ABI version (ASH_ABI_VERSION). */
+int ash_abi_version(void);
+
+/* Text of the last error raised on the calling thread. */
+const char* ash_last_error(void);
+
+/* Creates a plan; replaces the operator precompute of ASS:1301-1371
+ * (_build_operators / _to_jax).  Operators are evaluated on the fly, so the
+ * plan holds only 1-D tables; the field state needs nX*nY*MP*8 bytes. */
+int ash_plan_create(const ash_desc* desc, ash_plan** out);
+
+/* Releases the plan and its device workspace. */
+void ash_plan_destroy(ash_plan* plan);
+
+/* Sets the z step used by the following calls (ASS:1292-1296 and the partial
+ * last step of ASS:1447-1450, which rebuilds the operators at dZ_step). */
+int ash_plan_set_dz(ash_plan* plan, double dz);
+
+/* Uploads the initial field into the plan's spectral state (ASS:1408) and
+ * records, for the first step, the FP64 intensity I_before = sum_t p^2
+ * (ASS:1445) and max|p| into stats_dev[0] (ASS:1418-1420).  stats_dev may be
+ * NULL. */
+int ash_load_field(ash_plan* plan, const float* field_dev, uint32_t* stats_dev, void* stream);
+
+/* Writes the current real field (the np.array(field) of ASS:1567). */
+int ash_store_field(ash_plan* plan, float* field_dev, void* stream);
+
+/* One full z-step of the loop body ASS:1443-1535: the composite march step,
+ * frequency-weighted and super-absorbing boundaries, screens, sources and the
+ * pnp/ppp/pI/pIloss/pax reductions.  Plain steps (no screen, no source,
+ * io->dz equal to the plan dZ) replay one captured hipGraph. */
+int ash_step(ash_plan* plan, const ash_step_io* io, void* stream);
+
+/* ---- Operator-level entry points on real fields (parity tests) ---------- */
+
+/* march_step_split_kt / _split_tvd / _split_standard (ASS:883-913) or
+ * march_step_sequential (ASS:874-880), selected by the plan flags, at the
+ * plan dZ.  in_dev and out_dev may alias. */
+int ash_march_step(ash_plan* plan, const float* in_dev, float* out_dev, void* stream);
+
+/* _attenuation_step (ASS:667-685) with the filter of precalculate_ad_pow2 /
+ * precalculate_ad (ASS:523-592) at scale `scale` (0.5 = afilt3d_half). */
+int ash_attenuation_step(ash_plan* plan, const float* in_dev, float* out_dev, double scale, void* stream);
+
+/* _angular_spectrum_step (ASS:649-655) with HH (half=0) or HH_half (half=1)
+ * of precalculate_mas (ASS:396-462) and abl / abl_half (ASS:116-131). */
+int ash_angular_spectrum_step(ash_plan* plan, const float* in_dev, float* out_dev, int32_t half, void* stream);
+
+/* _kt_flux, _kt_flux_minmod, _rusanov_flux_standard, _rusanov_flux_tvd
+ * (ASS:688-868) on every (x, y) column, with step dz; subcycle =
+ * ASH_SUBCYCLE_COLUMN gives per-column CFL sub-cycling (cflTarget). */
+int ash_nonlinear_step(ash_plan* plan, const float* in_dev, float* out_dev, int32_t scheme, double dz,
+                       double beta_tvd, int32_t subcycle, void* stream);
+
+/* Global maximum of |field| (the float(jnp.max(jnp.abs(field))) of ASS:849),
+ * written as float bits into *out_bits_dev (must be zeroed by the caller). */
+int ash_max_abs(ash_plan* plan, const float* in_dev, uint32_t* out_bits_dev, void* stream);
+
+/* _freq_weighted_boundary_step (ASS:658-664) with precalculate_abl_freq. */
+int ash_freq_weighted_boundary_step(ash_plan* plan, const float* in_dev, float* out_dev, void* stream);
+
+/* _super_absorbing_step (ASS:185-272) with _make_boundary_weights. */
+int ash_super_absorbing_step(ash_plan* plan, const float* in_dev, float* out_dev, void* stream);
+
+/* _apply_phase_screen (ASS:352-390); amp_dev may be NULL. */
+int ash_apply_phase_screen(ash_plan* plan, const float* in_dev, float* out_dev, const float* phase_dev,
+                           const float* amp_dev, void* stream);
+
+/* Bytes of device workspace held by the plan. */
+int64_t ash_plan_workspace_bytes(const ash_plan* plan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
