@@ -2,6 +2,7 @@
 // and the per-step hipGraph.  Kernels live in kernels.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -93,6 +94,7 @@ struct ash_plan {
   int64_t ws_bytes = 0;
   bool loaded = false;
   int host_row = -1;
+  int sa_band = 0;  // last row index (from each edge) with a nonzero super-absorbing y weight
   // graph of a plain step
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -100,6 +102,11 @@ struct ash_plan {
   ash_step_io gkey{};
   double gkey_dz = 0.0;
   bool use_graph = true;
+  // per-kernel timing (ash_profile_step): events around each launch
+  bool prof_on = false;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_ev;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_next = 0;
 };
 
 namespace {
@@ -229,6 +236,7 @@ int build_tables(ash_plan* p) {
       }
       return w;
     };
+    p->sa_band = (int)std::min<long>(by, nY);
     if ((rc = upload(p, ramp(nX, (int)bx), &p->T.sa_wx))) return rc;
     if ((rc = upload(p, ramp(nY, (int)by), &p->T.sa_wy))) return rc;
     std::vector<float> cw(MP, 0.f);
@@ -242,6 +250,35 @@ int build_tables(ash_plan* p) {
   if ((rc = upload(p, twiddles(nX), &p->T.twx))) return rc;
   if ((rc = upload(p, twiddles(nY), &p->T.twy))) return rc;
   if ((rc = upload(p, twiddles(M), &p->T.twm))) return rc;
+  {
+    // pass-major twiddles of the column transform, [r * Ns + k] = W_{Ns R}^{k r}
+    auto radices = [&](bool inv) {
+      std::vector<int> r;
+      switch (M) {
+        case 128: r = {8, 4, 4}; break;
+        case 256: r = {8, 8, 4}; break;
+        case 512: r = {8, 8, 8}; break;
+        default: r = {16, 8, 8}; break;
+      }
+      if (inv) std::reverse(r.begin(), r.end());
+      return r;
+    };
+    std::vector<float2> tc;
+    for (int inv = 0; inv < 2; ++inv) {
+      std::vector<int> r = radices(inv);
+      int ns = r[0];
+      for (size_t pz = 1; pz < r.size(); ++pz) {
+        const int R = r[pz];
+        for (int rr = 0; rr < R; ++rr)
+          for (int k = 0; k < ns; ++k) {
+            const double a = -2.0 * M_PI * (double)k * rr / ((double)ns * R);
+            tc.push_back(make_float2((float)std::cos(a), (float)std::sin(a)));
+          }
+        ns *= R;
+      }
+    }
+    if ((rc = upload(p, tc, &p->T.twc))) return rc;
+  }
   {
     std::vector<float2> tn(M + 1);
     for (int k = 0; k <= M; ++k) {
@@ -278,8 +315,44 @@ OpDesc make_op(const ash_plan* p, bool prop, double dz_prop, bool att, double dz
   o.dz_att = dz_att;
   filter_band(p, dz_filter, &o.k_start, &o.k_max);
   o.norm = (float)(1.0 / ((double)p->d.nX * p->d.nY));
+  o.k_start2 = o.k_start * o.k_start;
+  o.k_max2 = o.k_max * o.k_max;
+  o.k_startf = (float)o.k_start;
+  o.inv_band = (float)(1.0 / (o.k_max - o.k_start));
+  o.dzp = (float)dz_prop;
   return o;
 }
+
+// ---- per-kernel timing -------------------------------------------------------
+hipEvent_t prof_event(ash_plan* p) {
+  if (p->ev_next == p->ev_pool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    p->ev_pool.push_back(e);
+  }
+  return p->ev_pool[p->ev_next++];
+}
+struct ProfScope {
+  ash_plan* p;
+  hipStream_t s;
+  int kind;
+  hipEvent_t a = nullptr;
+  ProfScope(ash_plan* p_, int kind_, hipStream_t s_) : p(p_), s(s_), kind(kind_) {
+    if (p->prof_on) {
+      a = prof_event(p);
+      if (a) (void)hipEventRecord(a, s);
+    }
+  }
+  ~ProfScope() {
+    if (p->prof_on && a) {
+      hipEvent_t b = prof_event(p);
+      if (b) {
+        (void)hipEventRecord(b, s);
+        p->prof_ev.push_back({kind, {a, b}});
+      }
+    }
+  }
+};
 
 // ---- launch helpers --------------------------------------------------------
 template <int L>
@@ -298,10 +371,38 @@ template <int L>
 void launch_ysa_L(ash_plan* p, const ScreenDesc& sd, hipStream_t s) {
   hipLaunchKernelGGL(k_ysa<L>, dim3(p->g.nX * p->g.nfb), dim3(LNT), 0, s, p->spec, p->T, p->g, sd);
 }
+template <int M, int MODE, int NLK>
+void launch_col_MN(ash_plan* p, const ColDesc& cd, hipStream_t s) {
+  const int ncols = p->g.nX * p->g.nY;
+  const int per_block = ColCfg<M>::WPB * ColCfg<M>::CPW;
+  int blocks = (ncols + per_block - 1) / per_block;
+  blocks = std::min(blocks, 2048);  // persistent: each wave walks several columns
+  hipLaunchKernelGGL((k_column<M, MODE, NLK>), dim3(blocks), dim3(256), 0, s, p->spec, p->T, p->g, cd);
+}
+template <int M, int MODE>
+void launch_col_MM(ash_plan* p, const ColDesc& cd, hipStream_t s) {
+  switch (cd.nl) {
+    case NL_KT: launch_col_MN<M, MODE, NL_KT>(p, cd, s); break;
+    case NL_KT_MINMOD: launch_col_MN<M, MODE, NL_KT_MINMOD>(p, cd, s); break;
+    case NL_RUSANOV: launch_col_MN<M, MODE, NL_RUSANOV>(p, cd, s); break;
+    case NL_TVD: launch_col_MN<M, MODE, NL_TVD>(p, cd, s); break;
+    default: launch_col_MN<M, MODE, NL_NONE>(p, cd, s); break;
+  }
+}
 template <int M>
 void launch_col_M(ash_plan* p, const ColDesc& cd, hipStream_t s) {
-  const int ncols = p->g.nX * p->g.nY;
-  hipLaunchKernelGGL(k_column<M>, dim3((ncols + CPB - 1) / CPB), dim3(CNT), 0, s, p->spec, p->T, p->g, cd);
+  switch (cd.mode) {
+    case COL_LOAD: launch_col_MN<M, COL_LOAD, NL_NONE>(p, cd, s); break;
+    case COL_NL: launch_col_MM<M, COL_NL>(p, cd, s); break;
+    case COL_TAPER: launch_col_MN<M, COL_TAPER, NL_NONE>(p, cd, s); break;
+    case COL_FINAL:
+      if (cd.n_src > 0 || cd.smask) launch_col_MN<M, COL_FINAL_SRC, NL_NONE>(p, cd, s);
+      else launch_col_MN<M, COL_FINAL, NL_NONE>(p, cd, s);
+      break;
+    case COL_STORE: launch_col_MN<M, COL_STORE, NL_NONE>(p, cd, s); break;
+    case COL_RAWNL: launch_col_MM<M, COL_RAWNL>(p, cd, s); break;
+    default: break;
+  }
 }
 
 #define DISPATCH_L(n, CALL)                    \
@@ -314,15 +415,28 @@ void launch_col_M(ash_plan* p, const ColDesc& cd, hipStream_t s) {
     default: break;                            \
   }
 
-void xfwd(ash_plan* p, hipStream_t s) { DISPATCH_L(p->g.nX, launch_xfwd_L<L>(p, s)); }
-void ypass(ash_plan* p, const OpDesc& op, hipStream_t s) { DISPATCH_L(p->g.nY, launch_ypass_L<L>(p, op, s)); }
+void xfwd(ash_plan* p, hipStream_t s) {
+  ProfScope ps(p, ASH_K_XFWD, s);
+  DISPATCH_L(p->g.nX, launch_xfwd_L<L>(p, s));
+}
+void ypass(ash_plan* p, const OpDesc& op, hipStream_t s) {
+  ProfScope ps(p, ASH_K_YPASS, s);
+  DISPATCH_L(p->g.nY, launch_ypass_L<L>(p, op, s));
+}
 void xinv(ash_plan* p, int epi, const EpiDesc& e, hipStream_t s) {
+  ProfScope ps(p, ASH_K_XINV, s);
   if (epi == 0) { DISPATCH_L(p->g.nX, (launch_xinv_L<L, 0>(p, e, s))); }
   else if (epi == 1) { DISPATCH_L(p->g.nX, (launch_xinv_L<L, 1>(p, e, s))); }
   else { DISPATCH_L(p->g.nX, (launch_xinv_L<L, 2>(p, e, s))); }
 }
-void ysa(ash_plan* p, const ScreenDesc& sd, hipStream_t s) { DISPATCH_L(p->g.nY, launch_ysa_L<L>(p, sd, s)); }
+void ysa(ash_plan* p, const ScreenDesc& sd, hipStream_t s) {
+  ProfScope ps(p, ASH_K_YSA, s);
+  DISPATCH_L(p->g.nY, launch_ysa_L<L>(p, sd, s));
+}
 void column(ash_plan* p, const ColDesc& cd, hipStream_t s) {
+  const int kind = cd.mode == COL_NL ? ASH_K_COL_NL : cd.mode == COL_TAPER ? ASH_K_COL_TAPER
+                 : cd.mode == COL_FINAL ? ASH_K_COL_FINAL : ASH_K_COL_OTHER;
+  ProfScope ps(p, kind, s);
   switch (p->g.M) {
     case 128: launch_col_M<128>(p, cd, s); break;
     case 256: launch_col_M<256>(p, cd, s); break;
@@ -415,6 +529,7 @@ void enqueue_step(ash_plan* p, const ash_step_io* io, hipStream_t s) {
   sd.invdy = (float)(1.0 / p->d.dY);
   const double f0_bin = p->d.f0 / (1.0 / (p->d.nT * p->d.dT));
   sd.f0b = (float)std::fmax(f0_bin, 1.0);
+  sd.band = p->sa_band;
   if (p->d.useSuperAbsorbing || io->n_screens > 0) {
     sd.sa = p->d.useSuperAbsorbing;
     if (io->n_screens > 0) { sd.phase = io->screen_phase_dev[0]; sd.amp = io->screen_amp_dev[0]; }
@@ -528,6 +643,7 @@ void ash_plan_destroy(ash_plan* p) {
   if (p->graph) (void)hipGraphDestroy(p->graph);
   if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
   for (void* a : p->allocs) (void)hipFree(a);
+  for (hipEvent_t e : p->ev_pool) (void)hipEventDestroy(e);
   delete p;
 }
 
@@ -691,6 +807,7 @@ int ash_super_absorbing_step(ash_plan* p, const float* in, float* out, void* str
   xinv(p, 2, e, s);
   ScreenDesc sd{};
   sd.sa = 1;
+  sd.band = p->sa_band;
   sd.sa_strength = (float)p->d.superAbsorbingStrength;
   sd.inv2dy = (float)(1.0 / (2 * p->d.dY));
   sd.invdy = (float)(1.0 / p->d.dY);
@@ -716,6 +833,36 @@ int ash_apply_phase_screen(ash_plan* p, const float* in, float* out, const float
   store_real(p, out, s);
   p->loaded = true;
   return check_launch();
+}
+
+int ash_profile_step(ash_plan* p, const ash_step_io* io, int32_t n_iter, float* ms_out, int32_t* count_out,
+                     void* stream) {
+  if (!p || !io || !ms_out || !count_out || n_iter <= 0) return fail(ASH_E_ARG, "bad argument");
+  if (!p->loaded) return fail(ASH_E_STATE, "ash_profile_step before ash_load_field");
+  if (io->row < 0 || io->row + n_iter > io->max_rows) return fail(ASH_E_ARG, "rows out of range");
+  hipStream_t s = (hipStream_t)stream;
+  for (int k = 0; k < ASH_K_COUNT; ++k) { ms_out[k] = 0.f; count_out[k] = 0; }
+  p->prof_ev.clear();
+  p->ev_next = 0;
+  p->prof_on = true;
+  hipLaunchKernelGGL(k_set_row, dim3(1), dim3(1), 0, s, p->d_row, io->row);
+  for (int it = 0; it < n_iter; ++it) enqueue_step(p, io, s);
+  p->prof_on = false;
+  p->host_row = io->row + n_iter;
+  int rc = check_launch();
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(s));
+  for (auto& e : p->prof_ev) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e.second.first, e.second.second));
+    ms_out[e.first] += ms;
+    count_out[e.first] += 1;
+  }
+  for (int k = 0; k < ASH_K_COUNT; ++k) {
+    if (count_out[k]) ms_out[k] /= (float)count_out[k];
+    count_out[k] /= n_iter;  // launches per step
+  }
+  return ASH_OK;
 }
 
 int64_t ash_plan_workspace_bytes(const ash_plan* p) { return p ? p->ws_bytes : 0; }

@@ -115,6 +115,7 @@ struct Tables {
   const float2* twy;     // [nY]
   const float2* twm;     // [M]  exp(-2 pi i k / M)
   const float2* twn;     // [M+1] exp(-2 pi i k / nT)
+  const float2* twc;     // column-transform pass twiddles: forward passes, then inverse passes
 };
 
 struct Geom {
@@ -131,61 +132,66 @@ struct OpDesc {
   double dz_att;
   double k_start, k_max;  // transverse cosine filter band
   float norm;             // 1/(nX nY) of the inverse 2-D FFT
+  double k_start2, k_max2;  // squared band edges
+  float k_startf, inv_band, dzp;
 };
 
-// Propagator x attenuation at natural bin (i, j, m); kk = kx^2 + ky^2 (FP64).
-__device__ __forceinline__ float2 visit_operator(const OpDesc& op, double kk, double kt, double ka,
-                                                 double al, double as, int m) {
-  float2 out = make_float2(op.norm, 0.f);
+// Per-f constants of one visit, staged in LDS by the y-pass.
+struct LineConst {
+  double kt2, ka2;   // squared ASM and attenuation wavenumbers (FP64)
+  float kt, ka;      // the same, single precision
+  float al, as;      // alpha(f) dz_att, alphaStar(f) dz_att
+  int kept;          // propagator keeps this bin
+};
+
+// Propagator x attenuation at natural bin (kx, ky, m); kk = kx^2 + ky^2 in FP64.
+// Only the differences kt^2 - kk and ka^2 - kk need FP64 (they cancel near the
+// evanescent cutoff); everything else is single precision.  The propagator
+// phase dz (kt - kz) = dz kk / (kt + kz) avoids the cancellation of kt - kz,
+// and the attenuation phase is folded into one sincos with it.
+__device__ __forceinline__ float2 visit_operator(const OpDesc& op, double kk, const LineConst& lc, int m) {
+  float ph = 0.f, lg = 0.f, amp = op.norm;
+  const float kkf = (float)kk;
   if (op.use_prop) {
-    if (kt <= 0.0) return make_float2(0.f, 0.f);  // DC, Nyquist, negative and padding bins
-    const double ktr = sqrt(kk);
-    float filt = 1.f;
-    if (ktr > op.k_max) return make_float2(0.f, 0.f);
-    if (ktr > op.k_start) {
-      const float pos = (float)((ktr - op.k_start) / (op.k_max - op.k_start));
-      filt = 0.5f * (1.f + cospif(pos));
+    if (!lc.kept || kk > op.k_max2) return make_float2(0.f, 0.f);
+    if (kk > op.k_start2) {
+      const float pos = (sqrtf(kkf) - op.k_startf) * op.inv_band;
+      amp *= 0.5f * (1.f + cospif(pos));
     }
-    const double d2 = kt * kt - kk;  // FP64: cancels near the evanescent cutoff
-    float ph, amp = filt;
-    if (d2 > 0.0) {
-      const float kz = sqrtf((float)d2);
-      ph = (float)op.dz_prop * ((float)kk / ((float)kt + kz));  // dz (kt - kz)
+    const float d2 = (float)(lc.kt2 - kk);
+    if (d2 > 0.f) {
+      ph = op.dzp * (kkf / (lc.kt + sqrtf(d2)));
     } else {
-      ph = (float)(op.dz_prop * kt);
-      amp *= expf(-(float)op.dz_prop * sqrtf((float)(-d2)));
+      ph = op.dzp * lc.kt;
+      lg = -op.dzp * sqrtf(-d2);
     }
-    float s, c;
-    sincosf(ph, &s, &c);
-    out = make_float2(out.x * amp * c, out.x * amp * s);
   }
   if (op.use_att && m > 0) {
     float re, im;
     bool apply = true;
     if (op.obliquity) {
-      const double kz2 = ka * ka - kk;
-      if (kz2 > 0.0) {
-        const double kz = fmax(sqrt(kz2), ka * 1e-6);
-        const double obl = ka / kz;
-        re = (float)(-al * op.dz_att * obl);
-        im = (float)(-as * op.dz_att * obl);
+      const float kz2 = (float)(lc.ka2 - kk);
+      if (kz2 > 0.f) {
+        const float obl = lc.ka / fmaxf(sqrtf(kz2), lc.ka * 1e-6f);
+        re = -lc.al * obl;
+        im = -lc.as * obl;
       } else {
         apply = false;
       }
     } else {
-      re = (float)(-al * op.dz_att);
-      im = (float)(-as * op.dz_att);
+      re = -lc.al;
+      im = -lc.as;
     }
     if (apply) {
-      const float a = expf(re);
-      float s, c;
-      sincosf(im, &s, &c);
-      float2 fac = make_float2(a * c, a * s);
-      if (op.clip_negative && fac.x < 0.f) fac = make_float2(0.f, 0.f);
-      out = cmul(out, fac);
+      if (op.clip_negative && cosf(im) < 0.f) return make_float2(0.f, 0.f);
+      lg += re;
+      ph += im;
     }
   }
-  return out;
+  if (lg != 0.f) amp *= expf(lg);
+  float s, c;
+  sincosf(ph, &s, &c);
+  return make_float2(amp * c, amp * s);
 }
 
 // ---------------------------------------------------------------------------
@@ -221,8 +227,8 @@ template <int L>
 __global__ __launch_bounds__(LNT) void k_ypass(float2* __restrict__ spec, Tables T, Geom g, OpDesc op) {
   __shared__ float2 tw[L];
   __shared__ float2 buf[L * LF];
-  __shared__ double sky[L];
-  __shared__ double skt[LF], ska[LF], sal[LF], sas[LF];
+  __shared__ double sky2[L];
+  __shared__ LineConst slc[LF];
   const int tid = threadIdx.x;
   const int x = blockIdx.x / g.nfb, fb = blockIdx.x % g.nfb;
   const int f0 = fb * LF;
@@ -230,14 +236,21 @@ __global__ __launch_bounds__(LNT) void k_ypass(float2* __restrict__ spec, Tables
   const long sy = g.MP;
   const int nvalid = min(LF, g.M + 1 - f0);
   load_tw<L>(tw, T.twy, tid, LNT);
-  for (int i = tid; i < L; i += LNT) sky[i] = T.ky[i];
-  if (tid < LF) {
-    const int m = min(f0 + tid, g.MP - 1);
-    skt[tid] = T.kt[m]; ska[tid] = T.ka[m]; sal[tid] = T.alpha[m]; sas[tid] = T.astar[m];
-  }
-  __syncthreads();
   const double kx = T.kx[x];
   const double kx2 = kx * kx;
+  for (int i = tid; i < L; i += LNT) sky2[i] = kx2 + T.ky[i] * T.ky[i];
+  if (tid < LF) {
+    const int m = min(f0 + tid, g.MP - 1);
+    const double kt = T.kt[m], ka = T.ka[m];
+    LineConst lc;
+    lc.kt2 = kt * kt; lc.ka2 = ka * ka;
+    lc.kt = (float)kt; lc.ka = (float)ka;
+    lc.al = (float)(T.alpha[m] * op.dz_att);
+    lc.as = (float)(T.astar[m] * op.dz_att);
+    lc.kept = kt > 0.0;
+    slc[tid] = lc;
+  }
+  __syncthreads();
   using LY = Lay<L, LF, 0, true>;
   auto ld = [&](int line, int pos) -> float2 {
     return line < nvalid ? base[pos * sy + line] : make_float2(0.f, 0.f);
@@ -247,8 +260,7 @@ __global__ __launch_bounds__(LNT) void k_ypass(float2* __restrict__ spec, Tables
   // the inverse pass 0 reads back from LDS through the operator
   auto ld_op = [&](int line, int pos) -> float2 {
     const float2 v = buf[LY::idx(line, pos)];
-    const double kk = kx2 + sky[pos] * sky[pos];
-    const float2 o = visit_operator(op, kk, skt[line], ska[line], sal[line], sas[line], f0 + line);
+    const float2 o = visit_operator(op, sky2[pos], slc[line], f0 + line);
     return cmul(v, o);
   };
   auto st = [&](int line, int pos, float2 v) {
@@ -363,6 +375,7 @@ struct ScreenDesc {
   int sa;
   float sa_strength;
   float inv2dy, invdy;
+  int band;            // rows 0..band-1 (and mirrored) carry nonzero super-absorbing weight
 };
 
 template <int L>
@@ -374,29 +387,35 @@ __global__ __launch_bounds__(LNT) void k_ysa(float2* __restrict__ spec, Tables T
   float2* base = spec + (long)x * g.nY * g.MP + f0;
   const long sy = g.MP;
   const int nvalid = min(LF, g.M + 1 - f0);
-  constexpr int PER = (L * LF) / LNT;
-  float2 v[PER];
-#pragma unroll
-  for (int q = 0; q < PER; ++q) {
-    const int el = tid + q * LNT;
-    const int line = el % LF, pos = el / LF;
-    v[q] = line < nvalid ? base[pos * sy + line] : make_float2(0.f, 0.f);
-    buf[pos * LF + line] = v[q];
+  // rows handled: all of them with a screen, else only the two boundary bands
+  // (rows 0..band and L-1-band..L-1, which hold every row with a nonzero
+  // super-absorbing weight and its neighbours)
+  const bool band = s.phase == nullptr && 2 * (s.band + 1) < L;
+  const int nrows = band ? 2 * (s.band + 1) : L;
+  auto row_of = [&](int i) { return (band && i > s.band) ? L - nrows + i : i; };
+  const int nel = nrows * LF;
+  for (int el = tid; el < nel; el += LNT) {
+    const int line = el % LF, i = el / LF;
+    buf[i * LF + line] = line < nvalid ? base[row_of(i) * sy + line] : make_float2(0.f, 0.f);
   }
   __syncthreads();
+  constexpr int PER = (L * LF) / LNT;
+  float2 res[PER];
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
     const int el = tid + q * LNT;
-    const int line = el % LF, pos = el / LF;
+    if (el >= nel) break;
+    const int line = el % LF, i = el / LF;
+    const int pos = row_of(i);
     const int m = f0 + line;
-    float2 u = v[q];
+    float2 u = buf[i * LF + line];
     if (s.sa) {
       const float w = T.sa_wy[pos];
       if (w != 0.f) {
         float2 d;
-        if (pos == 0) d = cscale(csub(buf[1 * LF + line], u), s.invdy);
-        else if (pos == L - 1) d = cscale(csub(u, buf[(L - 2) * LF + line]), s.invdy);
-        else d = cscale(csub(buf[(pos + 1) * LF + line], buf[(pos - 1) * LF + line]), s.inv2dy);
+        if (pos == 0) d = cscale(csub(buf[(i + 1) * LF + line], u), s.invdy);
+        else if (pos == L - 1) d = cscale(csub(u, buf[(i - 1) * LF + line]), s.invdy);
+        else d = cscale(csub(buf[(i + 1) * LF + line], buf[(i - 1) * LF + line]), s.inv2dy);
         const float cw = T.sa_cw[min(m, g.MP - 1)];
         const float2 cd = make_float2(cw * d.y, -cw * d.x);
         const float2 pin = pos < L / 2 ? cscale(cadd(u, cd), 0.5f) : cscale(csub(u, cd), 0.5f);
@@ -417,13 +436,14 @@ __global__ __launch_bounds__(LNT) void k_ysa(float2* __restrict__ spec, Tables T
       }
       if (m == 0 || m == g.M) u.y = 0.f;
     }
-    v[q] = u;
+    res[q] = u;
   }
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
     const int el = tid + q * LNT;
-    const int line = el % LF, pos = el / LF;
-    if (line < nvalid) base[pos * sy + line] = v[q];
+    if (el >= nel) break;
+    const int line = el % LF, i = el / LF;
+    if (line < nvalid) base[row_of(i) * sy + line] = res[q];
   }
 }
 
@@ -437,6 +457,7 @@ enum ColMode : int {
   COL_FINAL = 3,   // sources + reductions (+ spectrum rewrite if a source was added)
   COL_STORE = 4,   // spectrum -> real field
   COL_RAWNL = 5,   // real field in -> nonlinear step -> real field out (no FFT)
+  COL_FINAL_SRC = 6,  // COL_FINAL on a step that injects source slices
 };
 
 enum NLKind : int { NL_NONE = 0, NL_KT = 1, NL_RUSANOV = 2, NL_TVD = 3, NL_KT_MINMOD = 4 };
@@ -521,283 +542,11 @@ __device__ __forceinline__ void kt_rhs_local(const float* u, int t0, int nT, flo
   }
 }
 
-template <int M>
-__global__ __launch_bounds__(CNT) void k_column(float2* __restrict__ spec, Tables T, Geom g, ColDesc d) {
-  constexpr int NT_ = 2 * M;
-  constexpr int LS = M + 1;
-  constexpr int S = NT_ / TPC;  // real samples per thread
-  __shared__ float2 twm[M];
-  __shared__ float2 twn[M + 1];
-  __shared__ float2 buf[CPB * LS];
-  __shared__ float redmax[CPB];
-  const int tid = threadIdx.x;
-  const int col = tid / TPC, lane = tid % TPC;
-  const int ncols = g.nX * g.nY;
-  const long c = (long)blockIdx.x * CPB + col;
-  const bool live = c < ncols;
-  for (int i = tid; i < M; i += CNT) twm[i] = T.twm[i];
-  for (int i = tid; i <= M; i += CNT) twn[i] = T.twn[i];
-  float2* cbuf = buf + col * LS;
-  float* rb = reinterpret_cast<float*>(cbuf);  // the real column aliases the spectrum buffer
-  float2* sc = spec + c * g.MP;
-  using LY = Lay<M, CPB, LS, false>;
-  const float invN = 1.f / (float)NT_;
+}  // namespace ash
 
-  // ---- stage 1: obtain the real column in rb[0..nT) ----
-  if (d.mode == COL_LOAD || d.mode == COL_RAWNL) {
-    const float* src = d.rin + c * NT_;
-    for (int i = lane; i < NT_; i += TPC) rb[i] = live ? src[i] : 0.f;
-    __syncthreads();
-  } else {
-    for (int k = lane; k <= M; k += TPC) {
-      float2 v = live ? sc[k] : make_float2(0.f, 0.f);
-      if (k == 0 || k == M) v.y = 0.f;
-      cbuf[k] = v;
-    }
-    __syncthreads();
-    // c2r: Z[k] = (X_k + conj X_{M-k}) + i W_N^{-k} (X_k - conj X_{M-k}), then inverse M-point FFT
-    auto ldz = [&](int line, int k) -> float2 {
-      const float2* cb = buf + line * LS;
-      const float2 a = cb[k], b = cconj(cb[M - k]);
-      const float2 s = cadd(a, b), df = csub(a, b);
-      const float2 w = cconj(twn[k]);
-      const float2 t = cmul(w, df);
-      return make_float2((s.x - t.y) * 0.5f, (s.y + t.x) * 0.5f);
-    };
-    auto str = [&](int line, int n, float2 v) { buf[line * LS + n] = cscale(v, 2.f * invN); };
-    fft_run<M, true, CNT, CPB, LY>(buf, twm, tid, ldz, str);
-    __syncthreads();
-  }
+#include "column.h"
 
-  // ---- stage 2: real-space work on S contiguous samples per lane ----
-  const int t0 = lane * S;
-  float u[S];
-#pragma unroll
-  for (int i = 0; i < S; ++i) u[i] = rb[t0 + i];
-  bool rewrite = true;  // whether the spectrum is written back
-
-  if (d.mode == COL_NL || d.mode == COL_TAPER || d.mode == COL_RAWNL) {
-    if (d.taper) {
-      const float* tt = d.taper == 1 ? T.tapt_h : T.tapt_f;
-#pragma unroll
-      for (int i = 0; i < S; ++i) u[i] *= tt[t0 + i];
-    }
-    if (d.nl != NL_NONE && (d.mode == COL_NL || d.mode == COL_RAWNL)) {
-      int nsub = 1;
-      float dz = (float)d.dz;
-      if (d.subcycle) {
-        float mx = 0.f;
-#pragma unroll
-        for (int i = 0; i < S; ++i) mx = fmaxf(mx, fabsf(u[i]));
-        for (int o = TPC / 2; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, TPC));
-        const double cfl = d.N * (double)mx * d.dz / d.dT;
-        if (!(cfl <= (double)d.cfl_target || mx < 1e-30f)) nsub = (int)ceil(cfl / (double)d.cfl_target);
-        dz = (float)(d.dz / nsub);
-      }
-      // barriers inside the loop need a block-uniform trip count: run the
-      // block maximum and let columns that are done idle through the rest
-      __shared__ int snsub[CPB];
-      if (lane == 0) snsub[col] = nsub;
-      __syncthreads();
-      int nsub_blk = snsub[0];
-      for (int k = 1; k < CPB; ++k) nsub_blk = max(nsub_blk, snsub[k]);
-      const float coef = (float)(d.N / d.dT);
-      for (int sub = 0; sub < nsub_blk; ++sub) {
-        const bool act = sub < nsub;
-        if (d.nl == NL_KT || d.nl == NL_KT_MINMOD) {
-          // SSP-RK2 (ASS:833-839)
-          float h[S + 4], r0[S], s1[S];
-          __syncthreads();
-#pragma unroll
-          for (int i = 0; i < S; ++i) rb[t0 + i] = u[i];
-          __syncthreads();
-#pragma unroll
-          for (int i = 0; i < S + 4; ++i) {
-            const int t = t0 - 2 + i;
-            h[i] = (t >= 0 && t < NT_) ? rb[t] : 0.f;
-          }
-          if (d.nl == NL_KT) kt_rhs_local<S, true>(h, t0, NT_, coef, r0);
-          else kt_rhs_local<S, false>(h, t0, NT_, coef, r0);
-#pragma unroll
-          for (int i = 0; i < S; ++i) s1[i] = act ? u[i] + dz * r0[i] : u[i];
-          __syncthreads();
-#pragma unroll
-          for (int i = 0; i < S; ++i) rb[t0 + i] = s1[i];
-          __syncthreads();
-#pragma unroll
-          for (int i = 0; i < S + 4; ++i) {
-            const int t = t0 - 2 + i;
-            h[i] = (t >= 0 && t < NT_) ? rb[t] : 0.f;
-          }
-          if (d.nl == NL_KT) kt_rhs_local<S, true>(h, t0, NT_, coef, r0);
-          else kt_rhs_local<S, false>(h, t0, NT_, coef, r0);
-          if (act) {
-#pragma unroll
-            for (int i = 0; i < S; ++i) u[i] = 0.5f * u[i] + 0.5f * (s1[i] + dz * r0[i]);
-          }
-        } else {
-          // forward-Euler Rusanov (ASS:688-724)
-          float h[S + 2];
-          __syncthreads();
-#pragma unroll
-          for (int i = 0; i < S; ++i) rb[t0 + i] = u[i];
-          __syncthreads();
-#pragma unroll
-          for (int i = 0; i < S + 2; ++i) {
-            const int t = t0 - 1 + i;
-            h[i] = (t >= 0 && t < NT_) ? rb[t] : 0.f;
-          }
-          const float cf = (float)(d.N * (double)dz / d.dT);
-          float fl[S + 1];
-#pragma unroll
-          for (int i = 0; i < S + 1; ++i) {
-            const float uL = h[i], uR = h[i + 1];
-            const float a = fmaxf(fabsf(uL), fabsf(uR));
-            if (d.nl == NL_RUSANOV) fl[i] = -(uL * uL + uR * uR) / 2.f - a * (uR - uL);
-            else fl[i] = 0.5f * (-0.5f * (uL * uL + uR * uR)) - 0.5f * a * (uR - uL);
-          }
-#pragma unroll
-          for (int i = 0; i < S; ++i) {
-            const int t = t0 + i;
-            if (act && t >= 1 && t <= NT_ - 2) {
-              const float fd = fl[i + 1] - fl[i];
-              if (d.nl == NL_RUSANOV) {
-                u[i] += -cf * fd;
-              } else {
-                const float dm = h[i + 1] - h[i], dp = h[i + 2] - h[i + 1];
-                const float eps = 1e-30f, b = d.beta_tvd;
-                const float r = dp / (dm + eps * sgnf(dm + eps));
-                const float ri = dm / (dp + eps * sgnf(dp + eps));
-                const float pp = fmaxf(0.f, fminf(fminf(b * r, 1.f), fminf(r, b)));
-                const float pm = fmaxf(0.f, fminf(fminf(b * ri, 1.f), fminf(ri, b)));
-                u[i] += -cf * (0.5f * (pp + pm) * fd);
-              }
-            }
-          }
-        }
-      }
-    }
-  }
-
-  if (d.mode == COL_FINAL) {
-    rewrite = false;
-    const int row = *d.row;
-    if (d.n_src > 0 || d.smask) {
-      rewrite = true;
-      for (int k = 0; k < d.n_src; ++k) {
-        const float* sp = d.src[k] + c * NT_;
-#pragma unroll
-        for (int i = 0; i < S; ++i) u[i] += live ? sp[t0 + i] : 0.f;
-      }
-      if (d.smask && live && d.smask[c]) {
-        const double dl = d.sdelay[c];
-        const double w0 = 2.0 * 3.14159265358979323846 * d.s_f0;
-        const double cexp = 2.0 * d.s_dur;
-#pragma unroll
-        for (int i = 0; i < S; ++i) {
-          const double tm = d.taxis[t0 + i] - dl;
-          const double env = exp(-pow(1.05 * tm * w0 / (d.s_ncyc * 3.14159265358979323846), cexp));
-          u[i] += (float)(env * sin(w0 * tm) * d.s_p0);
-        }
-      }
-    }
-    // reductions over t (ASS:1529-1535); intensity in FP64
-    float mn = u[0], mx = u[0];
-    double e = 0.0;
-#pragma unroll
-    for (int i = 0; i < S; ++i) {
-      mn = fminf(mn, u[i]);
-      mx = fmaxf(mx, u[i]);
-      e += (double)u[i] * (double)u[i];
-    }
-    for (int o = TPC / 2; o > 0; o >>= 1) {
-      mn = fminf(mn, __shfl_xor(mn, o, TPC));
-      mx = fmaxf(mx, __shfl_xor(mx, o, TPC));
-      e += __shfl_xor(e, o, TPC);
-    }
-    if (live && row < d.max_rows) {
-      const long o = (long)row * ncols + c;
-      if (lane == 0) {
-        const double ib = d.ibefore[c];
-        d.pnp[o] = mn;
-        d.ppp[o] = mx;
-        d.pI[o] = (float)e;
-        d.pIloss[o] = (float)fmax(0.0, ib - e);
-        d.ibefore[c] = e;
-      }
-      if (c == d.cax) {
-#pragma unroll
-        for (int i = 0; i < S; ++i) d.pax[(long)row * NT_ + t0 + i] = u[i];
-      }
-    }
-    const float am = live ? fmaxf(mx, -mn) : 0.f;
-    if (lane == 0) redmax[col] = am;
-    __syncthreads();
-    if (tid == 0) {
-      float m = redmax[0];
-      for (int k = 1; k < CPB; ++k) m = fmaxf(m, redmax[k]);
-      if (row < d.max_rows) atomicMax(d.stats + row + 1, __float_as_uint(m));
-    }
-    if (d.rout && live) {
-#pragma unroll
-      for (int i = 0; i < S; ++i) d.rout[c * NT_ + t0 + i] = u[i];
-    }
-  } else if (d.mode == COL_LOAD) {
-    float mx = 0.f;
-    double e = 0.0;
-#pragma unroll
-    for (int i = 0; i < S; ++i) {
-      mx = fmaxf(mx, fabsf(u[i]));
-      e += (double)u[i] * (double)u[i];
-    }
-    for (int o = TPC / 2; o > 0; o >>= 1) {
-      mx = fmaxf(mx, __shfl_xor(mx, o, TPC));
-      e += __shfl_xor(e, o, TPC);
-    }
-    if (live && lane == 0 && d.ibefore) d.ibefore[c] = e;
-    if (lane == 0) redmax[col] = live ? mx : 0.f;
-    __syncthreads();
-    if (tid == 0 && d.stats) {
-      float m = redmax[0];
-      for (int k = 1; k < CPB; ++k) m = fmaxf(m, redmax[k]);
-      atomicMax(d.stats, __float_as_uint(m));
-    }
-  } else if (d.mode == COL_STORE) {
-    rewrite = false;
-    if (live) {
-#pragma unroll
-      for (int i = 0; i < S; ++i) d.rout[c * NT_ + t0 + i] = u[i];
-    }
-  } else if (d.mode == COL_RAWNL) {
-    rewrite = false;
-    if (live) {
-#pragma unroll
-      for (int i = 0; i < S; ++i) d.rout[c * NT_ + t0 + i] = u[i];
-    }
-  }
-  // block-uniform: the branch below contains barriers
-  bool any_rewrite = __syncthreads_or(rewrite ? 1 : 0);
-  if (!any_rewrite) return;
-
-  // ---- stage 3: r2c of the real column back into the spectrum ----
-#pragma unroll
-  for (int i = 0; i < S; ++i) rb[t0 + i] = u[i];
-  __syncthreads();
-  auto ldr = [&](int line, int k) -> float2 { return buf[line * LS + k]; };
-  auto stz = [&](int line, int k, float2 v) { buf[line * LS + k] = v; };
-  fft_run<M, false, CNT, CPB, LY>(buf, twm, tid, ldr, stz);
-  __syncthreads();
-  if (rewrite && live) {
-    for (int k = lane; k <= M; k += TPC) {
-      const float2 zk = cbuf[k == M ? 0 : k];
-      const float2 zm = cconj(cbuf[k == 0 ? 0 : M - k]);
-      const float2 s = cadd(zk, zm), df = csub(zk, zm);
-      const float2 t = cmul(twn[k], df);  // W_N^k (Z_k - conj Z_{M-k})
-      // X_k = s/2 - i t/2
-      sc[k] = make_float2(0.5f * (s.x + t.y), 0.5f * (s.y - t.x));
-    }
-  }
-}
+namespace ash {
 
 __global__ void k_set_row(int* row, int v) { *row = v; }
 __global__ void k_advance_row(int* row) { *row += 1; }

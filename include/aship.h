@@ -175,6 +175,26 @@ int ash_super_absorbing_step(ash_plan* plan, const float* in_dev, float* out_dev
 int ash_apply_phase_screen(ash_plan* plan, const float* in_dev, float* out_dev, const float* phase_dev,
                            const float* amp_dev, void* stream);
 
+/* Kernel kinds reported by ash_profile_step */
+#define ASH_K_XFWD 0        /* forward FFT along x */
+#define ASH_K_YPASS 1       /* FFT along y, operator multiply, inverse FFT */
+#define ASH_K_XINV 2        /* inverse FFT along x + epilogue */
+#define ASH_K_YSA 3         /* super-absorbing y pass / screen */
+#define ASH_K_COL_NL 4      /* t column: irfft, taper, nonlinear step, rfft */
+#define ASH_K_COL_TAPER 5   /* t column: irfft, taper, rfft */
+#define ASH_K_COL_FINAL 6   /* t column: irfft, sources, reductions */
+#define ASH_K_COL_OTHER 7
+#define ASH_K_COUNT 8
+
+/* Timing aid with no counterpart in the reference (whose only timing is the
+ * wall clock of ASS:1411, :1552): runs n_iter steps like ash_step, without the
+ * graph, with HIP events recorded on `stream` around every kernel launch.
+ * ms_out[k] receives the average duration (ms) of one launch of kind k and
+ * count_out[k] the launches of kind k per step.  Rows io->row ..
+ * io->row + n_iter - 1 of the outputs are written. */
+int ash_profile_step(ash_plan* plan, const ash_step_io* io, int32_t n_iter, float* ms_out,
+                     int32_t* count_out, void* stream);
+
 /* Bytes of device workspace held by the plan. */
 int64_t ash_plan_workspace_bytes(const ash_plan* plan);
 
