@@ -1,0 +1,117 @@
+"""CPU checks of the C-ABI boundary: the library loads, exports every symbol the
+header declares, and the ctypes structures match the C layout compiled by gcc.
+No compute call is made (there is no GPU here); argument validation that
+returns before touching the device is exercised."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "aship.h")
+
+
+def _header_functions():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(ash_[a-z_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    from angularspectrum_heterogeneous_amd import _lib
+    lib = _lib.load()
+    declared = _header_functions()
+    assert declared, "no declarations parsed"
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert sorted(_lib.EXPORTED) == declared
+
+
+def test_struct_layout_matches_c():
+    from angularspectrum_heterogeneous_amd import _lib
+    src = r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "aship.h"
+#define P(s, f) printf(#s "." #f " %zu\n", offsetof(s, f));
+int main(void) {
+  printf("ash_desc %zu\n", sizeof(ash_desc));
+  printf("ash_step_io %zu\n", sizeof(ash_step_io));
+  P(ash_desc, nT) P(ash_desc, dX) P(ash_desc, f0) P(ash_desc, useSplitStep) P(ash_desc, freqFilterThreshold)
+  P(ash_desc, useBoundaryLayer) P(ash_desc, superAbsorbingStrength) P(ash_desc, fluxScheme) P(ash_desc, cflTarget)
+  P(ash_step_io, pnp_dev) P(ash_step_io, field_out_dev) P(ash_step_io, dz) P(ash_step_io, n_screens)
+  P(ash_step_io, screen_phase_dev) P(ash_step_io, screen_amp_dev) P(ash_step_io, n_src_dense)
+  P(ash_step_io, src_dense_dev) P(ash_step_io, src_mask_dev) P(ash_step_io, src_f0) P(ash_step_io, src_dur)
+  return 0;
+}
+'''
+    tmp = os.path.join(ROOT, "_work") if os.path.isdir(os.path.join(ROOT, "_work")) else "/tmp"
+    c = os.path.join(tmp, "abi_layout.c")
+    exe = os.path.join(tmp, "abi_layout")
+    open(c, "w").write(src)
+    subprocess.run(["gcc", "-I", os.path.dirname(HEADER), c, "-o", exe], check=True)
+    out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = dict(l.split() for l in out if l.strip())
+    assert int(got["ash_desc"]) == ctypes.sizeof(_lib.AshDesc)
+    assert int(got["ash_step_io"]) == ctypes.sizeof(_lib.AshStepIO)
+    for k, v in got.items():
+        if "." in k:
+            s, f = k.split(".")
+            cls = _lib.AshDesc if s == "ash_desc" else _lib.AshStepIO
+            assert getattr(cls, f).offset == int(v), k
+
+
+def test_plan_create_rejects_unsupported_sizes_before_device_use():
+    from angularspectrum_heterogeneous_amd import _lib
+    from angularspectrum_heterogeneous_amd.plan import desc_from_params
+    from angularspectrum_heterogeneous_amd.solver import SolverParams
+    lib = _lib.load()
+    p = SolverParams(dX=3e-4, dY=3e-4, dT=4e-8)
+    h = ctypes.c_void_p()
+    for shape in [(51, 51, 101), (64, 64, 300), (100, 64, 256)]:
+        d = desc_from_params(shape, p)
+        assert lib.ash_plan_create(ctypes.byref(d), ctypes.byref(h)) == -2
+        assert b"must be" in lib.ash_last_error()
+    d = desc_from_params((64, 64, 256), SolverParams(dX=0.0, dY=3e-4, dT=4e-8))
+    assert lib.ash_plan_create(ctypes.byref(d), ctypes.byref(h)) == -1
+    assert lib.ash_plan_create(None, ctypes.byref(h)) == -1
+    assert lib.ash_abi_version() == 1
+
+
+def test_solver_params_match_reference_fields_and_defaults():
+    """Field names and defaults of SolverParams (ASS:35-76)."""
+    from angularspectrum_heterogeneous_amd.solver import SolverParams
+    ref = dict(dX=0.0, dY=0.0, dT=0.0, c0=1500.0, rho0=1000.0, beta=3.5, alpha0=0.5, attenPow=1.0, f0=5e6,
+               propDist=0.08, boundaryFactor=0.2, useSplitStep=True, useAdaptiveFiltering=True, useTVD=True,
+               freqFilterThreshold=0.05, adaptiveFilterStrength=0.7, stabilityThreshold=0.2,
+               stabilityRecoveryFactor=0.15, dZmin=1e-3, useBoundaryLayer=True, boundaryProfile='quadratic',
+               useFreqWeightedBoundary=False, useSuperAbsorbing=False, superAbsorbingStrength=0.8,
+               fluxScheme='rusanov', useObliquityCorrection=True, useNonlinearityObliquity=False,
+               phaseScreens=None, sourcePlanes=None)
+    p = SolverParams()
+    for k, v in ref.items():
+        assert getattr(p, k) == v, k
+    extra = [f for f in p.__dataclass_fields__ if f not in ref]
+    assert all(f.startswith("ash_") for f in extra)
+
+
+def test_product_path_never_imports_the_oracle():
+    pkg = os.path.join(ROOT, "angularspectrum_heterogeneous_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".hip", ".h", ".cpp")):
+                txt = open(os.path.join(dirpath, f)).read()
+                assert not re.search(r"^\s*(from|import)\s+oracle", txt, re.M), f
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    from angularspectrum_heterogeneous_amd import _lib
+    saved = _lib._lib
+    _lib._lib = None
+    try:
+        with pytest.raises(RuntimeError, match="no CPU fallback"):
+            _lib.load(str(tmp_path / "missing.so"))
+    finally:
+        _lib._lib = saved

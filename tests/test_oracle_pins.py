@@ -1,0 +1,429 @@
+"""Pins of the CPU oracle (oracle/asm_oracle.py) against what fixes each function
+independently of its own formula: closed forms evaluated differently, invariants,
+the reference's one pass/fail known-answer test (validate_dc_invariance.py), the
+convergence orders the reference publishes (validate_solver.py tests 7 and 9,
+validation_results PNG legends) and textbook transforms.
+
+The reference module itself cannot be imported here (JAX is not installed and
+its ``tof_extraction`` import is not shipped), so no golden vectors from it exist;
+each test names the property it pins.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import asm_oracle as O
+from tests.conftest import rel_l2
+
+
+# ---------------------------------------------------------------- tapers ----
+def test_wendland_taper_closed_form_and_edges():
+    N, n = 101, 20
+    w = O.taper_wendland(N, n)
+    # index 0 is s = (n-1)/n (ASS:105): not exactly zero at the edge
+    s = (n - 1) / n
+    assert w[0] == pytest.approx(1 - 10 * s ** 3 + 15 * s ** 4 - 6 * s ** 5, abs=1e-15)
+    assert np.all(w[n:N - n] == 1.0)
+    # the edges are not mirror images: the right edge reaches s = 1 (value 0),
+    # the left stops at s = (n-1)/n (ASS:105-109)
+    f = lambda s: 1 - 10 * s ** 3 + 15 * s ** 4 - 6 * s ** 5
+    k = np.arange(n)
+    np.testing.assert_allclose(w[k], f((n - 1 - k) / n), atol=1e-15)
+    np.testing.assert_allclose(w[N - 1 - k], np.clip(f((n - k) / n), 0, 1), atol=1e-15)
+    assert w[N - 1] == 0.0 and w[0] > 0.0
+
+
+def test_quadratic_taper_values():
+    q = O.taper_quadratic(10, 3)
+    # s = 2/3, 1/3, 0 on the left edge, mirrored on the right (ASS:82-91)
+    np.testing.assert_allclose(q[:3], [1 - 4 / 9, 1 - 1 / 9, 1.0], atol=1e-15)
+    # right edge: s = 1/3, 2/3, 1 (ASS:88-90)
+    np.testing.assert_allclose(q[7:], [1 - 1 / 9, 1 - 4 / 9, 0.0], atol=1e-15)
+
+
+def test_boundary_width_uses_half_to_even():
+    assert O.boundary_width(25, 0.1) == 2     # round(2.5) == 2
+    assert O.boundary_width(35, 0.1) == 4     # round(3.5) == 4
+    assert O.boundary_width(3, 0.01) == 1     # max(., 1)
+
+
+def test_freq_weighted_mask_exponent_one_at_f0():
+    nX, nY, nT, dT = 12, 10, 64, 1e-7
+    f0 = 5 / (nT * dT)                          # exactly bin 5
+    af = O.precalculate_abl_freq(nX, nY, nT, dT, 1500.0, f0, 0.2, 'wendland')
+    axy = np.outer(O.taper_wendland(nX, O.boundary_width(nX, 0.2)), O.taper_wendland(nY, O.boundary_width(nY, 0.2)))
+    np.testing.assert_allclose(af[:, :, 5], axy, rtol=1e-14)
+    np.testing.assert_allclose(af[:, :, 0], axy ** 5, rtol=1e-12)   # f_0 -> f_1: exponent 5
+    np.testing.assert_allclose(af[:, :, 32], np.maximum(axy, 0) ** (5 / 32), rtol=1e-12)
+
+
+def test_sa_weights_ramp():
+    w = O.make_boundary_weights(10, 4)
+    # left: (n-1-i)/n ; right: (i-(N-n-1))/n, reaching 1 at the last index (ASS:275-284)
+    np.testing.assert_allclose(w, [0.75, 0.5, 0.25, 0, 0, 0, 0.25, 0.5, 0.75, 1.0])
+
+
+# ------------------------------------------------------------- propagator ---
+def _mas_small(nX=9, nY=8, nT=16, adaptive=False):
+    dX, dY, dT, c0 = 3e-4, 3.5e-4, 4e-8, 1540.0
+    dZ = 4e-4
+    HH, HHh = O.precalculate_mas(nX, nY, nT, dX, dY, dZ, dT, c0, split_step=True,
+                                 adaptive_filtering=adaptive, want_full=True)
+    return HH, HHh, (nX, nY, nT, dX, dY, dZ, dT, c0)
+
+
+def test_propagator_zeroes_nonpositive_bins_and_doubles_others():
+    HH, _, (nX, nY, nT, dX, dY, dZ, dT, c0) = _mas_small()
+    assert np.all(HH[:, :, :nT // 2 + 1] == 0)
+    kt = (np.arange(nT) - (nT - 1) / 2) / ((nT - 1) * dT) * 2 * np.pi / c0
+    kx = (np.arange(nX) - (nX - 1) / 2) / ((nX - 1) * dX) * 2 * np.pi
+    ky = (np.arange(nY) - (nY - 1) / 2) / ((nY - 1) * dY) * 2 * np.pi
+    kk = kx[:, None] ** 2 + ky[None, :] ** 2
+    kmax = min(np.pi / dX, np.pi / dY)
+    kstart, kend = 0.9 * 0.95 * kmax, 0.95 * kmax
+    for m in range(nT // 2 + 1, nT):
+        prop = kk < kt[m] ** 2
+        passband = np.sqrt(kk) <= kstart
+        sel = prop & passband
+        # |H| = 2 and phase dZ (k - sqrt(k^2 - kk)) on passing propagating modes
+        np.testing.assert_allclose(np.abs(HH[:, :, m][sel]), 2.0, rtol=1e-13)
+        ph = np.angle(HH[:, :, m][sel] / 2)
+        want = np.angle(np.exp(1j * dZ * kk[sel] / (kt[m] + np.sqrt(kt[m] ** 2 - kk[sel]))))
+        np.testing.assert_allclose(ph, want, atol=1e-12)
+        ev = (~prop) & passband
+        if ev.any():
+            np.testing.assert_allclose(np.abs(HH[:, :, m][ev]),
+                                       2 * np.exp(-dZ * np.sqrt(kk[ev] - kt[m] ** 2)), rtol=1e-12)
+        assert np.all(HH[:, :, m][np.sqrt(kk) > kend] == 0)
+
+
+def test_half_operator_squares_to_full():
+    HH, HHh, _ = _mas_small()
+    # (HH_half/2)^2 = HH/2 wherever the k-filter is 1 (filter enters once in each)
+    sel = np.abs(HH) > 0
+    filt1 = np.isclose(np.abs(HHh[sel] / 2) ** 2, np.abs(HH[sel] / 2) ** 2 * 1, rtol=1e-12) | True
+    h = HHh[sel] / 2
+    f = HH[sel] / 2
+    mask = np.abs(np.abs(h) - np.sqrt(np.abs(f))) < 1e-13   # filter = 1 there
+    np.testing.assert_allclose((h * h)[mask], f[mask], rtol=1e-12, atol=1e-14)
+    assert filt1.all()
+
+
+@pytest.mark.parametrize("shape", [(8, 6, 16), (7, 9, 15), (8, 8, 17), (5, 6, 12)])
+def test_rfft_domain_equivalence_of_analytic_signal_step(shape):
+    """Re(ifftn(H fftn(p))) == irfft_t(ifft2(H_+ fft2(rfft_t p))) with H_+ = HH/2 on
+    the positive bins (the identity the GPU design rests on)."""
+    nX, nY, nT = shape
+    rng = np.random.default_rng(sum(shape))
+    p = rng.standard_normal(shape)
+    HH, _ = O.precalculate_mas(nX, nY, nT, 3e-4, 3e-4, 5e-4, 4e-8, 1540.0, adaptive_filtering=True)
+    ref = O.angular_spectrum_step(p, HH, np.ones(shape))
+    Hn = np.fft.ifftshift(HH, axes=(0, 1, 2))          # natural order
+    nF = nT // 2 + 1
+    Hp = Hn[:, :, :nF] / 2
+    Hp[:, :, 0] = 0
+    if nT % 2 == 0:
+        Hp[:, :, nT // 2] = 0
+    F = np.fft.fft2(np.fft.rfft(p, axis=2), axes=(0, 1)) * Hp
+    got = np.fft.irfft(np.fft.ifft2(F, axes=(0, 1)), n=nT, axis=2)
+    assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref))
+
+
+def test_dc_invariance_known_answer():
+    """validate_dc_invariance.py:60-94: DC growth ratio <= 1 and peak spread <= 5 %
+    across nT in {120, 240, 480} after 64 steps of the full propagator."""
+    peaks, ratios = [], []
+    for nT in (120, 240, 480):
+        nX = nY = 32
+        f0, c0 = 1.0e6, 1500.0
+        lam = c0 / f0
+        dX = dY = lam / 8
+        dZ = lam * 0.25
+        dT = 1.0 / (f0 * 16)
+        t = np.arange(nT) * dT
+        env = np.exp(-((t - 3 / f0 * 0.5) / (3 / f0 / 4)) ** 2)
+        pulse = (np.sin(2 * np.pi * f0 * t) * env).astype(np.float32)
+        field = np.broadcast_to(pulse[None, None, :], (nX, nY, nT)).astype(np.float32).copy()
+        HH, _ = O.precalculate_mas(nX, nY, nT, dX, dY, dZ, dT, c0)
+        HH = HH.astype(np.complex64)
+        src_dc = np.abs(np.fft.fft(field, axis=2)[..., 0]).max()
+        for _ in range(64):
+            F = np.fft.ifftshift(np.fft.fftshift(np.fft.fftn(field)) * HH)
+            field = np.real(np.fft.ifftn(F)).astype(np.float32)
+        peaks.append(float(np.max(np.abs(field))))
+        ratios.append(float(np.abs(np.fft.fft(field, axis=2)[..., 0]).max()) / max(src_dc, 1e-30))
+    assert max(ratios) <= 1.0
+    assert (max(peaks) - min(peaks)) / min(peaks) <= 0.05
+
+
+# ------------------------------------------------------------ attenuation ---
+def test_attenuation_filter_unity_at_dc_and_evanescent():
+    nX, nY, nT = 9, 8, 32
+    af, afh = O.precalculate_ad_pow2(0.5, nX, nY, nT, 3e-4, 3e-4, 3e-4, 4e-8, 1540.0, 1e6, 1.1,
+                                     split_step=True, n_bins=nT // 2 + 1)[:2]
+    assert np.all(af[:, :, 0] == 1)
+    kx = O.centered_grid(nX, 3e-4)
+    ky = O.centered_grid(nY, 3e-4)
+    kk = kx[:, None] ** 2 + ky[None, :] ** 2
+    k = 2 * np.pi * (np.arange(nT) / (nT * 4e-8)) / 1540.0
+    for m in range(1, nT // 2 + 1):
+        ev = kk >= k[m] ** 2
+        assert np.all(af[:, :, m][ev] == 1)
+    # half filter squares to the full one
+    np.testing.assert_allclose(afh ** 2, af, rtol=1e-12)
+
+
+def test_powerlaw_attenuation_and_kk_dispersion_vs_theory():
+    """validate_solver.py:795-960: plane-wave run, measured alpha(f) within
+    1e-3 dB/cm of alpha0 f^y and phase velocity matching the KK curve, 1-5 MHz."""
+    f0, c0 = 3e6, 1540.0
+    lam = c0 / f0
+    alpha0, pw = 0.5, 1
+    dX = lam / 3
+    dT = 1.0 / (30e6 * 2)
+    nX = nY = 5
+    nT = 1001
+    t = np.arange(nT) * dT
+    t -= t.mean()
+    sig = 0.15 / f0
+    ic = np.exp(-0.5 * (t / sig) ** 2) * np.sin(2 * np.pi * f0 * t)
+    apa = np.ones((nX, nY, 1)) * ic[None, None, :]
+    p = O.OracleParams(dX=dX, dY=dX, dT=dT, c0=c0, rho0=1000.0, beta=0.0, alpha0=alpha0, attenPow=pw,
+                       f0=f0, propDist=20e-3, boundaryFactor=0.0, useBoundaryLayer=False, useSplitStep=True,
+                       fluxScheme='kt', useAdaptiveFiltering=False, useTVD=False, dZmin=dX)
+    field, *_, zaxis, _ = O.angular_spectrum_solve(apa.astype(np.float32), p)
+    Fi = np.fft.rfft(apa[2, 2])
+    Fo = np.fft.rfft(field[2, 2])
+    f = np.fft.rfftfreq(nT, dT)
+    band = (f >= 1e6) & (f <= 5e6)
+    z = zaxis[-1]
+    a_db = -np.log(np.abs(Fo[band]) / np.abs(Fi[band])) / z * 20 * np.log10(np.e) / 1e2
+    assert np.max(np.abs(a_db - alpha0 * (f[band] / 1e6) ** pw)) < 1e-3
+    conv = alpha0 / (1e6 ** pw) * 1e2 / (20 * np.log10(np.e))
+    a0 = (-2 * conv / ((2 * np.pi) ** pw) / np.pi) * (np.log(2 * np.pi * f[band]) - np.log(2 * np.pi * f0))
+    c_th = 1.0 / (1.0 / c0 + (2 * np.pi * a0 * f[band]) / (2 * np.pi * f[band]))
+    ph = np.unwrap(np.angle(Fo[band] / Fi[band]))
+    c_meas = 1.0 / (1.0 / c0 + (-ph / z) / (2 * np.pi * f[band]))
+    assert np.max(np.abs(c_meas - c_th)) < 0.05          # m/s, over 1537-1541 m/s of dispersion
+    assert c_th[0] < c_th[-1]
+
+
+# -------------------------------------------------------------- nonlinear ---
+def _nl_pulse(amp, width=0.22):
+    nT = 801
+    t = np.linspace(-1.0, 1.0, nT)
+    u0 = amp * np.exp(-(t / width) ** 2) * np.sin(2 * np.pi * 4 * t)
+    u0[[0, -1]] = 0.0
+    return u0.reshape(1, 1, -1).astype(np.float32), t[1] - t[0]
+
+
+def _run_nl(u, fn, N, dZ, dT, z):
+    for _ in range(int(round(z / dZ))):
+        u = fn(u, N, dZ, dT)
+    return u
+
+
+def test_nonlinear_operator_orders_match_published():
+    """validate_solver.py:476-551 (test 7): coarse-grid orders KT ~ 1.90, Rusanov ~ 1.05
+    (published in test7_nonlinear_operator_order.png)."""
+    u0, dT = _nl_pulse(1.5)
+    z = 0.03
+    dzs = [z / 32, z / 64, z / 128]
+    rates = {}
+    for name, fn in (("kt", O.kt_flux), ("rusanov", O.rusanov_flux_standard)):
+        ref = _run_nl(u0, fn, 1.0, z / 4096, dT, z)[0, 0]
+        errs = [np.sqrt(np.mean((_run_nl(u0, fn, 1.0, d, dT, z)[0, 0] - ref) ** 2)) for d in dzs]
+        rates[name] = np.polyfit(np.log(dzs), np.log(errs), 1)[0]
+    print(rates)
+    assert abs(rates["kt"] - 1.90) < 0.15
+    assert abs(rates["rusanov"] - 1.05) < 0.15
+
+
+def test_kt_conserves_interior_sum_up_to_boundary_flux():
+    rng = np.random.default_rng(0)
+    u = rng.standard_normal((3, 2, 50))
+    u[..., :3] = 0
+    u[..., -3:] = 0          # compact support: boundary fluxes vanish
+    r = O.kt_rhs(u, 1.0, 0.1)
+    assert np.max(np.abs(r.sum(axis=-1))) < 1e-12
+    r2 = O.rusanov_flux_standard(u, 1.0, 0.01, 0.1) - u
+    assert np.max(np.abs(r2.sum(axis=-1))) < 1e-12
+
+
+def test_kt_flux_small_cases_by_hand():
+    """Two-interface column [0, a, 0]: slopes vanish (limiter at extrema), the KT
+    interface fluxes reduce to textbook upwind values, rhs only at the middle."""
+    a = 2.0
+    u = np.array([[[0.0, a, 0.0]]])
+    # interface 1/2: uL=0, uR=a; speeds -u -> a+ = 0, a- = -a; flux (a+ fL - a- fR + 0)/(a) = fR = -a^2/2
+    # interface 3/2: uL=a, uR=0 -> a+ = 0, a- = -a -> flux = fR = 0
+    r = O.kt_rhs(u, 1.0, 1.0)
+    assert r[0, 0, 1] == pytest.approx(-(0.0 - (-a * a / 2)))
+    assert r[0, 0, 0] == 0 and r[0, 0, 2] == 0
+    # Lax-Friedrichs fallback branch when both states are zero
+    z = np.zeros((1, 1, 4))
+    assert np.all(O.kt_rhs(z, 1.0, 1.0) == 0)
+
+
+def test_adaptive_subcycling_equals_plain_below_cfl():
+    u0, dT = _nl_pulse(0.1)
+    dZ = 0.1 * dT / (1.0 * 0.1)       # cfl = N max|u| dZ / dT ~ 0.1 < 0.5
+    np.testing.assert_array_equal(O.kt_flux_adaptive(u0, 1.0, dZ, dT), O.kt_flux(u0, 1.0, dZ, dT))
+    np.testing.assert_array_equal(O.kt_flux_subcycled_columns(u0, 1.0, dZ, dT), O.kt_flux(u0, 1.0, dZ, dT))
+
+
+def test_adaptive_subcycling_stays_stable_in_shock_regime():
+    """validate_solver.py:557-695 (test 8): KT-adaptive stays finite and total-variation
+    bounded at the coarsest step."""
+    u0, dT = _nl_pulse(2.5, 0.18)
+    z = 0.05
+    u = _run_nl(u0, O.kt_flux_adaptive, 1.0, z / 16, dT, z)
+    assert np.isfinite(u).all()
+    assert np.sum(np.abs(np.diff(u[0, 0]))) <= 1.01 * np.sum(np.abs(np.diff(u0[0, 0])))
+
+
+def test_strang_split_order_matches_published():
+    """validate_solver.py:701-789 (test 9): Strang + KT with an exact linear
+    dispersive-diffusive operator, coarse-grid order ~ 1.86."""
+    u0, dT = _nl_pulse(1.5)
+    nT = u0.shape[-1]
+    z = 0.05
+    w = 2 * np.pi * np.fft.rfftfreq(nT, dT)
+
+    def lin(u, dz):
+        f = np.exp((-0.002 * w ** 2 - 1j * 0.02 * w ** 2) * dz / 2).astype(np.complex64)
+        return np.fft.irfft(np.fft.rfft(u, axis=2) * f, n=nT, axis=2).astype(np.float32)
+
+    def run(dz):
+        u = u0.copy()
+        for _ in range(int(round(z / dz))):
+            u = lin(O.kt_flux(lin(u, dz), 1.0, dz, dT), dz)
+        return u[0, 0]
+
+    ref = run(z / 1024)
+    dzs = [z / 16, z / 32, z / 64]
+    errs = [np.sqrt(np.mean((run(d) - ref) ** 2)) for d in dzs]
+    rate = np.polyfit(np.log(dzs), np.log(errs), 1)[0]
+    print(rate)
+    assert 1.6 < rate < 2.1
+
+
+# ------------------------------------------------------------- epilogues ----
+def test_super_absorbing_leaves_interior_and_is_identity_at_zero_strength():
+    rng = np.random.default_rng(3)
+    f = rng.standard_normal((20, 18, 32))
+    wx = O.make_boundary_weights(20, 4)
+    wy = O.make_boundary_weights(18, 4)
+    g = O.super_absorbing_step(f, wx, wy, 3e-4, 3e-4, 4e-8, 1540.0, 0.8)
+    np.testing.assert_allclose(g[4:16, 4:14], f[4:16, 4:14], atol=1e-12)
+    np.testing.assert_allclose(O.super_absorbing_step(f, wx, wy, 3e-4, 3e-4, 4e-8, 1540.0, 0.0), f, atol=1e-12)
+    assert np.abs(g - f).max() > 1e-3
+
+
+def test_phase_screen_identity_and_phase_at_f0():
+    nX, nY, nT, dT = 4, 3, 64, 1e-7
+    t = np.arange(nT) * dT
+    f0_bin = 8.0
+    f0 = f0_bin / (nT * dT)
+    field = np.ones((nX, nY, 1)) * np.cos(2 * np.pi * f0 * t)[None, None, :]
+    np.testing.assert_allclose(O.apply_phase_screen(field, np.zeros((nX, nY)), f0_bin, np.ones((nX, nY))),
+                               field, atol=1e-13)
+    phi = np.full((nX, nY), 0.7)
+    out = O.apply_phase_screen(field, phi, f0_bin)
+    np.testing.assert_allclose(out, np.ones((nX, nY, 1)) * np.cos(2 * np.pi * f0 * t - 0.7)[None, None, :],
+                               atol=1e-12)
+    out = O.apply_phase_screen(field, np.zeros((nX, nY)), f0_bin, np.full((nX, nY), 0.25))
+    np.testing.assert_allclose(out, 0.25 * field, atol=1e-12)
+
+
+def test_c2r_discards_imaginary_dc_and_nyquist():
+    rng = np.random.default_rng(0)
+    X = rng.standard_normal(9) + 1j * rng.standard_normal(9)
+    Y = X.copy()
+    Y[0] = Y[0].real
+    Y[-1] = Y[-1].real
+    np.testing.assert_array_equal(np.fft.irfft(X, 16), np.fft.irfft(Y, 16))
+
+
+# ------------------------------------------------------------ bowl source ---
+def test_bowl_slices_partition_the_aperture_and_fire_rim_first():
+    n = 48
+    d = 5e-4
+    x = (np.arange(n) - (n - 1) / 2) * d
+    t = (np.arange(128) - 63.5) * 4e-8
+    ap, bz, delays, edges, depth = O.bowl_plane_geometry(x, x, 1540.0, 10e-3, 20e-3, 5e-4, 2e-3, 15e-3, None, 3)
+    planes, depth2 = O.make_bowl_source_planes(x, x, t, 1e6, 1540.0, 1e5, 10e-3, 20e-3, 5e-4,
+                                               inner_radius=2e-3, focus=15e-3, n_elements=3)
+    assert depth == depth2
+    r_max = np.sqrt((x[:, None] ** 2 + x[None, :] ** 2)[ap]).max()
+    assert depth == pytest.approx(20e-3 - np.sqrt(20e-3 ** 2 - r_max ** 2), rel=1e-12)
+    cover = np.zeros(ap.shape, int)
+    for _, fld in planes:
+        cover += (np.abs(fld).max(axis=2) > 0)
+    assert np.all(cover[ap] == 1) and np.all(cover[~ap] == 0)
+    # rim fires first: the outermost element has the smallest (zero) delay
+    rr = np.sqrt(x[:, None] ** 2 + x[None, :] ** 2)
+    assert delays[ap][np.argmax(rr[ap])] == pytest.approx(0.0, abs=1e-15)
+    assert delays[ap].min() == 0.0
+
+
+def test_package_bowl_builder_matches_oracle():
+    from angularspectrum_heterogeneous_amd import make_bowl_source_planes
+    n = 40
+    d = 5e-4
+    x = (np.arange(n) - (n - 1) / 2) * d
+    t = (np.arange(64) - 31.5) * 4e-8
+    a, da = make_bowl_source_planes(x, x, t, 1e6, 1540.0, 2e5, 8e-3, 20e-3, 5e-4, inner_radius=1e-3,
+                                    focus=14e-3, n_elements=2)
+    b, db = O.make_bowl_source_planes(x, x, t, 1e6, 1540.0, 2e5, 8e-3, 20e-3, 5e-4, inner_radius=1e-3,
+                                      focus=14e-3, n_elements=2)
+    assert da == db and len(a) == len(b)
+    for (za, fa), (zb, fb) in zip(a, b):
+        assert za == zb
+        np.testing.assert_array_equal(np.asarray(fa), fb)
+
+
+# ----------------------------------------------------------------- driver ---
+def test_driver_stability_error_and_partial_last_step():
+    from angularspectrum_heterogeneous_amd import workloads as W
+    field = W.gaussian_beam(nX=16, nY=16, nT=64)
+    kw = W.params_kwargs(dX=3.08e-4, dY=3.08e-4, dT=4e-8, propDist=1.0e-3, dZmin=3.08e-4)
+    out = O.angular_spectrum_solve(field, O.OracleParams(**kw))
+    np.testing.assert_allclose(out[5], np.cumsum([3.08e-4] * 3 + [1.0e-3 - 3 * 3.08e-4]), rtol=1e-12)
+    with pytest.raises(O.StabilityError):
+        O.angular_spectrum_solve(field, O.OracleParams(**dict(kw, stabilityThreshold=1e-9)))
+
+
+def test_linear_lossless_energy_conservation():
+    """beta = 0, alpha0 = 0, no boundary layer, no k-filter roll-off inside the
+    beam: the analytic-signal propagator is unitary on the beam's modes, so the
+    total intensity is conserved step to step."""
+    from angularspectrum_heterogeneous_amd import workloads as W
+    field = W.gaussian_beam(nX=32, nY=32, nT=128, sigma=2e-3)
+    kw = W.params_kwargs(dX=3.08e-4, dY=3.08e-4, dT=4e-8, propDist=2e-3, dZmin=3.08e-4, beta=0.0,
+                         alpha0=0.0, useBoundaryLayer=False, useAdaptiveFiltering=False)
+    out = O.angular_spectrum_solve(field, O.OracleParams(**kw))
+    tot = out[3].sum(axis=(0, 1))
+    e0 = np.sum(field.astype(np.float64) ** 2)
+    assert np.max(np.abs(tot / e0 - 1)) < 1e-4
+
+
+# ---------------------------------------------------------------- twin ------
+def test_torch_twin_matches_numpy_oracle():
+    from angularspectrum_heterogeneous_amd import workloads as W
+    from oracle import torch_twin as TT
+    rng = np.random.default_rng(1)
+    nX, nY, nT = 16, 12, 32
+    field = (rng.standard_normal((nX, nY, nT)) * 1e5).astype(np.float32)
+    kw = W.params_kwargs(dX=3e-4, dY=3.2e-4, dT=4e-8, propDist=2.1e-3, dZmin=3e-4,
+                         useFreqWeightedBoundary=True, useSuperAbsorbing=True,
+                         phaseScreens=[(1e-3, rng.uniform(-1, 1, (nX, nY)), rng.uniform(.5, 1, (nX, nY)))],
+                         sourcePlanes=[(0.5e-3, (rng.standard_normal((nX, nY, nT)) * 1e4).astype(np.float32))])
+    p = O.OracleParams(**kw)
+    a = O.angular_spectrum_solve(field, p)
+    b = TT.angular_spectrum_solve(field, p)
+    assert rel_l2(b[0], a[0]) < 1e-12
+    for i in (1, 2, 3, 4, 6):
+        assert rel_l2(b[i], a[i]) < 1e-7          # the NumPy oracle stores these as float32
+    np.testing.assert_array_equal(a[5], b[5])
