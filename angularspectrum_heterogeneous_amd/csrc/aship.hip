@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -94,7 +95,8 @@ struct ash_plan {
   int64_t ws_bytes = 0;
   bool loaded = false;
   int host_row = -1;
-  int sa_band = 0;  // last row index (from each edge) with a nonzero super-absorbing y weight
+  int sa_band = 0;
+  int precise_ops = 0;  // evaluate the k-space operators in FP64 (set from ASH_PRECISE_OPS)  // last row index (from each edge) with a nonzero super-absorbing y weight
   // graph of a plain step
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -320,6 +322,7 @@ OpDesc make_op(const ash_plan* p, bool prop, double dz_prop, bool att, double dz
   o.k_startf = (float)o.k_start;
   o.inv_band = (float)(1.0 / (o.k_max - o.k_start));
   o.dzp = (float)dz_prop;
+  o.precise = p->precise_ops;
   return o;
 }
 
@@ -611,6 +614,7 @@ int ash_plan_create(const ash_desc* desc, ash_plan** out) {
   p->g.MP = ((p->g.M + 1 + LF - 1) / LF) * LF;
   p->g.nfb = p->g.MP / LF;
   p->N = d.beta / (2 * d.c0 * d.c0 * d.c0 * d.rho0);
+  if (const char* ev = std::getenv("ASH_PRECISE_OPS")) p->precise_ops = std::atoi(ev);
   p->dz = 0.0;
   int rc = build_tables(p);
   if (rc) { ash_plan_destroy(p); return rc; }

@@ -38,16 +38,35 @@ __device__ __forceinline__ void dft2(float2& a, float2& b) {
   b = csub(t, b);
 }
 
+// Irrational butterfly constants as double-float pairs (hi + lo equals the
+// constant to ~1e-16).  Multiplying as fma(x, hi, x*lo) rounds once, without
+// bias.  With the constants rounded to float instead, sqrt(1/2), cos(pi/8) and
+// sin(pi/8) all round down, every radix-8/16 butterfly then loses energy by a
+// fixed factor (~3e-8 per path), and over the ~10^3 transforms of a march that
+// bias accumulates into a coherent gain error on the beam.
+constexpr float H_HI = 0.7071067690849304f, H_LO = 1.2101617485882343e-08f;   // sqrt(1/2)
+constexpr float C1_HI = 0.9238795042037964f, C1_LO = 2.830748968563057e-08f;  // cos(pi/8)
+constexpr float S1_HI = 0.3826834261417389f, S1_LO = 6.2233507236442165e-09f; // sin(pi/8)
+constexpr float S3_HI = 0.8660253882408142f, S3_LO = 1.5543625053737742e-08f; // sqrt(3)/2
+
+__device__ __forceinline__ float mc(float x, float hi, float lo) { return fmaf(x, hi, x * lo); }
+
+// w * (sp P + i sq Q), P and Q positive constants given as hi/lo pairs, sp, sq = +-1
+__device__ __forceinline__ float2 cmul_c(float2 w, float ph, float pl, float sp, float qh, float ql, float sq) {
+  const float xp = sp * mc(w.x, ph, pl), yq = sq * mc(w.y, qh, ql);
+  const float xq = sq * mc(w.x, qh, ql), yp = sp * mc(w.y, ph, pl);
+  return make_float2(xp - yq, xq + yp);
+}
+
 template <bool INV>
 __device__ __forceinline__ void dft3(float2& a, float2& b, float2& c) {
-  const float c1 = -0.5f;
-  const float s1 = INV ? 0.86602540378443864676f : -0.86602540378443864676f;
+  const float sg = INV ? 1.f : -1.f;  // sin(2 pi/3) sign
   float2 t1 = cadd(b, c);
   float2 t2 = csub(b, c);
   float2 y0 = cadd(a, t1);
-  float2 m1 = make_float2(a.x + c1 * t1.x, a.y + c1 * t1.y);
-  // (b - c) * i*s1
-  float2 m2 = make_float2(-s1 * t2.y, s1 * t2.x);
+  float2 m1 = make_float2(a.x - 0.5f * t1.x, a.y - 0.5f * t1.y);
+  // (b - c) * i * sg * sqrt(3)/2
+  float2 m2 = make_float2(-sg * mc(t2.y, S3_HI, S3_LO), sg * mc(t2.x, S3_HI, S3_LO));
   a = y0;
   b = cadd(m1, m2);
   c = csub(m1, m2);
@@ -65,16 +84,20 @@ __device__ __forceinline__ void dft4(float2& a, float2& b, float2& c, float2& d)
 
 template <bool INV>
 __device__ __forceinline__ void dft8(float2* v) {
-  const float h = 0.70710678118654752440f;
   // radix-2 first stage (decimation in frequency)
   float2 a0 = cadd(v[0], v[4]), a4 = csub(v[0], v[4]);
   float2 a1 = cadd(v[1], v[5]), a5 = csub(v[1], v[5]);
   float2 a2 = cadd(v[2], v[6]), a6 = csub(v[2], v[6]);
   float2 a3 = cadd(v[3], v[7]), a7 = csub(v[3], v[7]);
   // twiddles W8^1, W8^2, W8^3 on the odd half
-  a5 = INV ? make_float2(h * (a5.x - a5.y), h * (a5.x + a5.y)) : make_float2(h * (a5.x + a5.y), h * (a5.y - a5.x));
+  if (INV) {
+    a5 = make_float2(mc(a5.x - a5.y, H_HI, H_LO), mc(a5.x + a5.y, H_HI, H_LO));
+    a7 = make_float2(-mc(a7.x + a7.y, H_HI, H_LO), mc(a7.x - a7.y, H_HI, H_LO));
+  } else {
+    a5 = make_float2(mc(a5.x + a5.y, H_HI, H_LO), mc(a5.y - a5.x, H_HI, H_LO));
+    a7 = make_float2(mc(a7.y - a7.x, H_HI, H_LO), -mc(a7.x + a7.y, H_HI, H_LO));
+  }
   a6 = rot90<INV>(a6);
-  a7 = INV ? make_float2(-h * (a7.x + a7.y), h * (a7.x - a7.y)) : make_float2(h * (a7.y - a7.x), -h * (a7.x + a7.y));
   dft4<INV>(a0, a1, a2, a3);
   dft4<INV>(a4, a5, a6, a7);
   v[0] = a0; v[2] = a1; v[4] = a2; v[6] = a3;
@@ -84,9 +107,6 @@ __device__ __forceinline__ void dft8(float2* v) {
 template <bool INV>
 __device__ __forceinline__ void dft16(float2* v) {
   // 4 x 4 decomposition: n = 4 n1 + n2, k = k1 + 4 k2
-  const float c1 = 0.92387953251128675613f;  // cos(pi/8)
-  const float s1 = 0.38268343236508977173f;  // sin(pi/8)
-  const float h = 0.70710678118654752440f;
   float2 w[16];
 #pragma unroll
   for (int n2 = 0; n2 < 4; ++n2) {
@@ -94,14 +114,18 @@ __device__ __forceinline__ void dft16(float2* v) {
     dft4<INV>(a, b, c, d);
     w[n2 * 4 + 0] = a; w[n2 * 4 + 1] = b; w[n2 * 4 + 2] = c; w[n2 * 4 + 3] = d;
   }
-  // twiddle W16^(n2*k1)
+  // twiddle W16^(n2*k1); sg is the sign of the imaginary parts (forward: -)
   const float sg = INV ? 1.f : -1.f;
-  const float2 t1 = make_float2(c1, sg * s1), t2 = make_float2(h, sg * h), t3 = make_float2(s1, sg * c1);
-  const float2 t6 = make_float2(-h, sg * h), t9 = make_float2(-c1, -sg * s1);
-  // n2=1: k1=1,2,3 -> W^1, W^2, W^3 ; n2=2: W^2, W^4, W^6 ; n2=3: W^3, W^6, W^9
-  w[5] = cmul(w[5], t1); w[6] = cmul(w[6], t2); w[7] = cmul(w[7], t3);
-  w[9] = cmul(w[9], t2); w[10] = rot90<INV>(w[10]); w[11] = cmul(w[11], t6);
-  w[13] = cmul(w[13], t3); w[14] = cmul(w[14], t6); w[15] = cmul(w[15], t9);
+  // W^1 = (c1, sg s1), W^2 = (h, sg h), W^3 = (s1, sg c1), W^6 = (-h, sg h), W^9 = (-c1, -sg s1)
+  w[5] = cmul_c(w[5], C1_HI, C1_LO, 1.f, S1_HI, S1_LO, sg);
+  w[6] = cmul_c(w[6], H_HI, H_LO, 1.f, H_HI, H_LO, sg);
+  w[7] = cmul_c(w[7], S1_HI, S1_LO, 1.f, C1_HI, C1_LO, sg);
+  w[9] = cmul_c(w[9], H_HI, H_LO, 1.f, H_HI, H_LO, sg);
+  w[10] = rot90<INV>(w[10]);
+  w[11] = cmul_c(w[11], H_HI, H_LO, -1.f, H_HI, H_LO, sg);
+  w[13] = cmul_c(w[13], S1_HI, S1_LO, 1.f, C1_HI, C1_LO, sg);
+  w[14] = cmul_c(w[14], H_HI, H_LO, -1.f, H_HI, H_LO, sg);
+  w[15] = cmul_c(w[15], C1_HI, C1_LO, -1.f, S1_HI, S1_LO, -sg);
 #pragma unroll
   for (int k1 = 0; k1 < 4; ++k1) {
     float2 a = w[k1], b = w[4 + k1], c = w[8 + k1], d = w[12 + k1];

@@ -134,11 +134,14 @@ struct OpDesc {
   float norm;             // 1/(nX nY) of the inverse 2-D FFT
   double k_start2, k_max2;  // squared band edges
   float k_startf, inv_band, dzp;
+  int precise;              // evaluate in FP64 and round once (accuracy reference)
 };
 
 // Per-f constants of one visit, staged in LDS by the y-pass.
 struct LineConst {
   double kt2, ka2;   // squared ASM and attenuation wavenumbers (FP64)
+  double ktd;        // ASM wavenumber (FP64)
+  double ald, asd;   // alpha dz_att, alphaStar dz_att (FP64)
   float kt, ka;      // the same, single precision
   float al, as;      // alpha(f) dz_att, alphaStar(f) dz_att
   int kept;          // propagator keeps this bin
@@ -149,7 +152,39 @@ struct LineConst {
 // evanescent cutoff); everything else is single precision.  The propagator
 // phase dz (kt - kz) = dz kk / (kt + kz) avoids the cancellation of kt - kz,
 // and the attenuation phase is folded into one sincos with it.
+__device__ __noinline__ float2 visit_operator_fp64(const OpDesc& op, double kk, const LineConst& lc, int m) {
+  double ph = 0.0, lg = 0.0, amp = op.norm;
+  if (op.use_prop) {
+    if (!lc.kept || kk > op.k_max2) return make_float2(0.f, 0.f);
+    if (kk > op.k_start2) amp *= 0.5 * (1.0 + cospi((sqrt(kk) - op.k_start) / (op.k_max - op.k_start)));
+    const double d2 = lc.kt2 - kk;
+    if (d2 > 0.0) ph = op.dz_prop * (kk / (lc.ktd + sqrt(d2)));
+    else { ph = op.dz_prop * lc.ktd; lg = -op.dz_prop * sqrt(-d2); }
+  }
+  if (op.use_att && m > 0) {
+    const double ka = sqrt(lc.ka2);
+    double re = 0.0, im = 0.0;
+    bool apply = true;
+    if (op.obliquity) {
+      const double kz2 = lc.ka2 - kk;
+      if (kz2 > 0.0) {
+        const double obl = ka / fmax(sqrt(kz2), ka * 1e-6);
+        re = -lc.ald * obl; im = -lc.asd * obl;
+      } else apply = false;
+    } else { re = -lc.ald; im = -lc.asd; }
+    if (apply) {
+      if (op.clip_negative && cos(im) < 0.0) return make_float2(0.f, 0.f);
+      lg += re; ph += im;
+    }
+  }
+  amp *= exp(lg);
+  double s, c;
+  sincos(ph, &s, &c);
+  return make_float2((float)(amp * c), (float)(amp * s));
+}
+
 __device__ __forceinline__ float2 visit_operator(const OpDesc& op, double kk, const LineConst& lc, int m) {
+  if (op.precise) return visit_operator_fp64(op, kk, lc, m);
   float ph = 0.f, lg = 0.f, amp = op.norm;
   const float kkf = (float)kk;
   if (op.use_prop) {
@@ -244,9 +279,12 @@ __global__ __launch_bounds__(LNT) void k_ypass(float2* __restrict__ spec, Tables
     const double kt = T.kt[m], ka = T.ka[m];
     LineConst lc;
     lc.kt2 = kt * kt; lc.ka2 = ka * ka;
+    lc.ktd = kt;
     lc.kt = (float)kt; lc.ka = (float)ka;
-    lc.al = (float)(T.alpha[m] * op.dz_att);
-    lc.as = (float)(T.astar[m] * op.dz_att);
+    lc.ald = T.alpha[m] * op.dz_att;
+    lc.asd = T.astar[m] * op.dz_att;
+    lc.al = (float)lc.ald;
+    lc.as = (float)lc.asd;
     lc.kept = kt > 0.0;
     slc[tid] = lc;
   }

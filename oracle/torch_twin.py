@@ -177,8 +177,28 @@ def march_step(field, ops, p, N, dZ):
     return attenuation_step(field, ops['afilt_half'])
 
 
-def angular_spectrum_solve(initial_field, p, device='cpu', max_steps_run=None):
-    """Driver of ASS:1217-1567 in float64 on ``device`` (same logic as the NumPy oracle)."""
+def source_slice_field(sp, device, dtype):
+    """Dense field of a mask + delay source slice (the tone burst of ASS:1193-1202),
+    synthesised on ``device`` in float64 and rounded to float32 as the reference
+    stores it."""
+    mask = torch.from_numpy(np.asarray(sp.mask)).to(device)
+    d = torch.from_numpy(np.asarray(sp.delays, dtype=np.float64)).to(device)
+    t = torch.from_numpy(np.asarray(sp.taxis, dtype=np.float64)).to(device)
+    w0 = 2 * np.pi * sp.f0
+    tm = t[None, None, :] - d[:, :, None]
+    env = torch.exp(-(1.05 * tm * w0 / (sp.ncycles * np.pi)) ** (2 * sp.dur))
+    fld = mask.to(torch.float64)[:, :, None] * env * torch.sin(w0 * tm) * sp.p0
+    return fld.to(torch.float32).to(dtype)
+
+
+def angular_spectrum_solve(initial_field, p, device='cpu', max_steps_run=None, dtype=np.float64,
+                           progress=0):
+    """Driver of ASS:1217-1567 on ``device`` (same logic as the NumPy oracle).
+
+    dtype float64: the FP64 reference; float32: the reference's own dtype flow
+    (operators built in FP64 and cast, ASS:1343-1351).  ``progress`` > 0 prints
+    a line every that many steps.
+    """
     init = np.asarray(initial_field)
     nX, nY, nT = init.shape
     c0 = p.c0
@@ -186,11 +206,12 @@ def angular_spectrum_solve(initial_field, p, device='cpu', max_steps_run=None):
     max_amp = np.max(np.abs(init))
     dZ = min(p.stabilityRecoveryFactor * p.dT / (max_amp * N + 1e-30), p.dZmin) if max_amp > 0 else p.dZmin
     dZ = max(dZ, p.dZmin)
-    ops = _ops_to(O.build_operators(nX, nY, nT, p, dZ, np.float64), device)
+    tdt = torch.float64 if np.dtype(dtype) == np.float64 else torch.float32
+    ops = _ops_to(O.build_operators(nX, nY, nT, p, dZ, dtype), device)
     abl_freq = None
     if p.useFreqWeightedBoundary:
         abl_freq = torch.from_numpy(O.precalculate_abl_freq(nX, nY, nT, p.dT, c0, p.f0, p.boundaryFactor,
-                                                            p.boundaryProfile)).to(device)
+                                                            p.boundaryProfile, dtype)).to(device)
     if p.useSuperAbsorbing:
         wx = O.make_boundary_weights(nX, O.boundary_width(nX, p.boundaryFactor)).astype(np.float64)
         wy = O.make_boundary_weights(nY, O.boundary_width(nY, p.boundaryFactor)).astype(np.float64)
@@ -198,7 +219,7 @@ def angular_spectrum_solve(initial_field, p, device='cpu', max_steps_run=None):
     max_steps = int(np.ceil(p.propDist / dZ)) + 100
     outs = torch.zeros((4, max_steps, nX, nY), dtype=torch.float64, device=device)
     pax = torch.zeros((max_steps, nT), dtype=torch.float64, device=device)
-    field = torch.from_numpy(init.astype(np.float64)).to(device)
+    field = torch.from_numpy(init.astype(np.float32).astype(dtype)).to(device)
     zvec = []
     cc = 0
     while sum(zvec) < p.propDist - 1e-15:
@@ -209,10 +230,10 @@ def angular_spectrum_solve(initial_field, p, device='cpu', max_steps_run=None):
         max_field = float(field.abs().max())
         if N * dZ_step / p.dT * max_field > p.stabilityThreshold:
             raise O.StabilityError('stability criterion violated')
-        I_before = (field ** 2).sum(dim=2)
+        I_before = (field.double() ** 2).sum(dim=2)
         step_ops = ops
         if abs(dZ_step - dZ) > 1e-15:
-            step_ops = _ops_to(O.build_operators(nX, nY, nT, p, dZ_step, np.float64), device)
+            step_ops = _ops_to(O.build_operators(nX, nY, nT, p, dZ_step, dtype), device)
         field = march_step(field, step_ops, p, N, dZ_step)
         if abl_freq is not None:
             field = freq_weighted_boundary_step(field, abl_freq)
@@ -222,22 +243,27 @@ def angular_spectrum_solve(initial_field, p, device='cpu', max_steps_run=None):
         if p.phaseScreens is not None:
             for scr in p.phaseScreens:
                 if z_cur < scr[0] <= z_cur + dZ_step:
-                    ph = torch.from_numpy(np.asarray(scr[1], dtype=np.float64)).to(device)
+                    ph = torch.from_numpy(np.asarray(scr[1]).astype(dtype)).to(device)
                     amp = None
                     if len(scr) > 2 and scr[2] is not None:
-                        amp = torch.from_numpy(np.asarray(scr[2], dtype=np.float64)).to(device)
+                        amp = torch.from_numpy(np.asarray(scr[2]).astype(dtype)).to(device)
                     field = apply_phase_screen(field, ph, f0_bin, amp)
         if p.sourcePlanes is not None:
             for sp_z, sp in p.sourcePlanes:
                 if z_cur < sp_z <= z_cur + dZ_step:
-                    field = field + torch.from_numpy(np.asarray(sp).astype(np.float32).astype(np.float64)).to(device)
-        I_after = (field ** 2).sum(dim=2)
+                    if hasattr(sp, 'mask') and hasattr(sp, 'delays'):
+                        field = field + source_slice_field(sp, device, tdt)
+                    else:
+                        field = field + torch.from_numpy(np.asarray(sp).astype(np.float32).astype(dtype)).to(device)
+        I_after = (field.double() ** 2).sum(dim=2)
         outs[3, cc] = torch.clamp(I_before - I_after, min=0.0)
         outs[2, cc] = I_after
-        outs[0, cc] = field.min(dim=2).values
-        outs[1, cc] = field.max(dim=2).values
-        pax[cc] = field[nX // 2, nY // 2, :]
+        outs[0, cc] = field.min(dim=2).values.double()
+        outs[1, cc] = field.max(dim=2).values.double()
+        pax[cc] = field[nX // 2, nY // 2, :].double()
         zvec.append(dZ_step)
         cc += 1
+        if progress and cc % progress == 0:
+            print(f'  twin step {cc} z={sum(zvec):.5f}', flush=True)
     res = outs[:, :cc].permute(0, 2, 3, 1).cpu().numpy()
-    return (field.cpu().numpy(), res[0], res[1], res[2], res[3], np.cumsum(zvec[:cc]), pax[:cc].t().cpu().numpy())
+    return (field.double().cpu().numpy(), res[0], res[1], res[2], res[3], np.cumsum(zvec[:cc]), pax[:cc].t().cpu().numpy())

@@ -24,7 +24,7 @@ def _compare(field, kw, tol=1e-4):
     t0 = time.perf_counter()
     got = angular_spectrum_solve(field, SolverParams(**kw), verbose=False)
     t1 = time.perf_counter()
-    ref = TT.angular_spectrum_solve(np.asarray(field), O.OracleParams(**kw), device="cuda")
+    ref = TT.angular_spectrum_solve(np.asarray(field), O.OracleParams(**kw), device="cuda", progress=50)
     t2 = time.perf_counter()
     torch.cuda.empty_cache()
     print(f"gpu solve {t1 - t0:.2f} s, fp64 twin {t2 - t1:.2f} s, {len(got[5])} steps", flush=True)
