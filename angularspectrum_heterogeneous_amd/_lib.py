@@ -95,6 +95,15 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         raise RuntimeError(
             f"{path} is missing: build it with angularspectrum_heterogeneous_amd._build.build() "
             "(hipcc --offload-arch=gfx950). There is no CPU fallback.")
+    # One HIP runtime per process: PyTorch's ROCm wheel bundles libamdhip64
+    # (SONAME libamdhip64.so.7).  Importing torch first registers that runtime,
+    # and this library's libamdhip64.so.7 dependency then binds to it; loaded
+    # the other way round, a second runtime would be mapped and the caller's
+    # hipStream_t handles would belong to a different runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(path)
     for name, (args, res) in _PROTOS.items():
         fn = getattr(lib, name)

@@ -364,7 +364,10 @@ void launch_xfwd_L(ash_plan* p, hipStream_t s) {
 }
 template <int L>
 void launch_ypass_L(ash_plan* p, const OpDesc& op, hipStream_t s) {
-  hipLaunchKernelGGL(k_ypass<L>, dim3(p->g.nX * p->g.nfb), dim3(LNT), 0, s, p->spec, p->T, p->g, op);
+  if (op.precise)
+    hipLaunchKernelGGL((k_ypass<L, true>), dim3(p->g.nX * p->g.nfb), dim3(LNT), 0, s, p->spec, p->T, p->g, op);
+  else
+    hipLaunchKernelGGL((k_ypass<L, false>), dim3(p->g.nX * p->g.nfb), dim3(LNT), 0, s, p->spec, p->T, p->g, op);
 }
 template <int L, int EPI>
 void launch_xinv_L(ash_plan* p, const EpiDesc& e, hipStream_t s) {

@@ -18,7 +18,10 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ int pidx(int n) { return n + (n >> 3); }
+#ifndef ASH_COL_PAD_SHIFT
+#define ASH_COL_PAD_SHIFT 3
+#endif
+__device__ __forceinline__ int pidx(int n) { return n + (n >> ASH_COL_PAD_SHIFT); }
 
 template <int M> struct ColRadix;
 template <> struct ColRadix<128>  { static constexpr int NP = 3, R0 = 8,  R1 = 4, R2 = 4; };
@@ -32,7 +35,7 @@ struct ColCfg {
   static constexpr int E = M / LPC;                          // complex samples per lane
   static constexpr int CPW = 64 / LPC;                       // columns per wave
   static constexpr int S = 2 * E;                            // real samples per lane
-  static constexpr int LDSN = M + (M >> 3) + 4;              // padded per-column LDS (float2)
+  static constexpr int LDSN = M + (M >> ASH_COL_PAD_SHIFT) + 4;  // padded per-column LDS (float2)
   static constexpr int WPB = 4;                              // waves per block
 };
 

@@ -152,7 +152,7 @@ struct LineConst {
 // evanescent cutoff); everything else is single precision.  The propagator
 // phase dz (kt - kz) = dz kk / (kt + kz) avoids the cancellation of kt - kz,
 // and the attenuation phase is folded into one sincos with it.
-__device__ __noinline__ float2 visit_operator_fp64(const OpDesc& op, double kk, const LineConst& lc, int m) {
+__device__ __forceinline__ float2 visit_operator_fp64(const OpDesc& op, double kk, const LineConst& lc, int m) {
   double ph = 0.0, lg = 0.0, amp = op.norm;
   if (op.use_prop) {
     if (!lc.kept || kk > op.k_max2) return make_float2(0.f, 0.f);
@@ -184,7 +184,6 @@ __device__ __noinline__ float2 visit_operator_fp64(const OpDesc& op, double kk, 
 }
 
 __device__ __forceinline__ float2 visit_operator(const OpDesc& op, double kk, const LineConst& lc, int m) {
-  if (op.precise) return visit_operator_fp64(op, kk, lc, m);
   float ph = 0.f, lg = 0.f, amp = op.norm;
   const float kkf = (float)kk;
   if (op.use_prop) {
@@ -258,7 +257,7 @@ __global__ __launch_bounds__(LNT) void k_xfft_fwd(float2* __restrict__ spec, Tab
 // y-pass: forward FFT along y, multiply by the visit operator, inverse FFT.
 // Tile = (kx, f-block).  The 1/(nX nY) normalisation is in the operator.
 // ---------------------------------------------------------------------------
-template <int L>
+template <int L, bool PRECISE>
 __global__ __launch_bounds__(LNT) void k_ypass(float2* __restrict__ spec, Tables T, Geom g, OpDesc op) {
   __shared__ float2 tw[L];
   __shared__ float2 buf[L * LF];
@@ -296,10 +295,15 @@ __global__ __launch_bounds__(LNT) void k_ypass(float2* __restrict__ spec, Tables
   auto st_lds = [&](int line, int pos, float2 v) { buf[LY::idx(line, pos)] = v; };
   fft_run<L, false, LNT, LF, LY>(buf, tw, tid, ld, st_lds);
   // the inverse pass 0 reads back from LDS through the operator
+  // with the line-fast butterfly mapping and LNT a multiple of LF, every
+  // butterfly of this thread is on line tid % LF: keep its constants in registers
+  static_assert(LNT % LF == 0, "the line of a thread must be fixed");
+  const int myline = tid % LF;
+  const LineConst lc = slc[myline];
   auto ld_op = [&](int line, int pos) -> float2 {
     const float2 v = buf[LY::idx(line, pos)];
-    const float2 o = visit_operator(op, sky2[pos], slc[line], f0 + line);
-    return cmul(v, o);
+    if constexpr (PRECISE) return cmul(v, visit_operator_fp64(op, sky2[pos], lc, f0 + myline));
+    else return cmul(v, visit_operator(op, sky2[pos], lc, f0 + myline));
   };
   auto st = [&](int line, int pos, float2 v) {
     if (line < nvalid) base[pos * sy + line] = v;
@@ -543,17 +547,22 @@ __device__ __forceinline__ float slope(float um, float u0, float up) {
   if (MC) return minmod3f(2.f * dm, 0.5f * (dm + dp), 2.f * dp);
   return minmod2f(dm, dp);
 }
-// KT central-upwind flux at an interface (ASS:771-793)
+// KT central-upwind flux at an interface (ASS:771-793).  When both local
+// speeds -u have one sign the formula reduces exactly to the upwind flux
+// (a- = 0 gives f-, a+ = 0 gives f+); only a sign change across the interface
+// needs the quotient, evaluated with the fast reciprocal.
 __device__ __forceinline__ float kt_iface(float umi, float upi) {
   const float ap = fmaxf(fmaxf(-umi, -upi), 0.f);
   const float am = fminf(fminf(-umi, -upi), 0.f);
   const float fm = -0.5f * umi * umi;
   const float fp = -0.5f * upi * upi;
   const float den = ap - am;
-  if (fabsf(den) < 1e-14f) {
+  if (den < 1e-14f) {
     return 0.5f * (fm + fp) - 0.5f * fmaxf(fabsf(umi), fabsf(upi)) * (upi - umi);
   }
-  return (ap * fm - am * fp - ap * am * (upi - umi)) / den;
+  if (am == 0.f) return fm;
+  if (ap == 0.f) return fp;
+  return __fdividef(ap * fm - am * fp - ap * am * (upi - umi), den);
 }
 
 // Semi-discrete KT rhs for S samples t0..t0+S-1 from u[-2..S+1] (index offset 2).

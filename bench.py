@@ -64,6 +64,8 @@ def main():
     ap.add_argument("--profile-iters", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run 3 device copies of 512 MiB (known HBM bytes) to scale PMC byte counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -136,6 +138,13 @@ def main():
     step_frac = step_bytes / (elapsed / K) / 1e9 / HBM_PEAK_GBS
 
     fin = float(stats[: row + 1].view(torch.float32).max().item())
+    if args.copy_calib:
+        a = torch.ones(128 * 1024 * 1024, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     result = {
         "metric": METRIC,
         "value": value,

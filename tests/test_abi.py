@@ -115,3 +115,14 @@ def test_missing_library_fails_loudly(tmp_path):
             _lib.load(str(tmp_path / "missing.so"))
     finally:
         _lib._lib = saved
+
+
+def test_single_hip_runtime_in_process():
+    """The library must share PyTorch's HIP runtime (it is handed torch's
+    streams and device pointers): after loading, one libamdhip64 is mapped."""
+    pytest.importorskip("torch")
+    from angularspectrum_heterogeneous_amd import _lib
+    _lib.load()
+    maps = open("/proc/self/maps").read()
+    libs = {l.split()[-1] for l in maps.splitlines() if "libamdhip64" in l}
+    assert len(libs) == 1, libs
