@@ -65,7 +65,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
-                    help="also run 3 device copies of 512 MiB (known HBM bytes) to scale PMC byte counters")
+                    help="also run 3 elementwise passes over 512 MiB (known HBM bytes) to scale PMC byte counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -134,15 +134,24 @@ def main():
     for name, kv in kernels.items():
         b = algorithmic_bytes_per_launch(name, vox, nX * nY, nT)
         kv["alg_GBps"] = round(b / (kv["avg_ms"] * 1e-3) / 1e9, 1)
+    # measured HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated
+    # on a kernel of known traffic), recorded next to the profile summaries
+    traffic = None
+    tp = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic_256x256x1024.json")
+    if os.path.exists(tp):
+        t = json.load(open(tp))["kernels"].get(dom)
+        if t:
+            traffic = t["traffic_bytes"]
     step_bytes = 104.0 * vox  # 13 sweeps x 8 B at this configuration
     step_frac = step_bytes / (elapsed / K) / 1e9 / HBM_PEAK_GBS
 
     fin = float(stats[: row + 1].view(torch.float32).max().item())
     if args.copy_calib:
+        # an elementwise kernel of known traffic: reads 512 MiB, writes 512 MiB
         a = torch.ones(128 * 1024 * 1024, dtype=torch.float32, device=dev)
         b = torch.empty_like(a)
         for _ in range(3):
-            b.copy_(a)
+            torch.add(a, 1.0, out=b)
         torch.cuda.synchronize()
         del a, b
     result = {
@@ -164,7 +173,7 @@ def main():
                    "nX": nX, "nY": nY, "nT": nT, "dZ": dz, "z_steps_per_s": K / elapsed,
                    "max_abs_p_seen": fin},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "alg_bytes_per_launch": ab},
         "step_roofline": {"alg_bytes_per_voxel_step": 104, "frac": round(step_frac, 4)},
         "kernels": kernels,
