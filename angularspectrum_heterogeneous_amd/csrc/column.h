@@ -147,6 +147,26 @@ __global__ __launch_bounds__(256) void k_column(float2* __restrict__ spec, Table
   const int t0 = ln * S;
   const float invN = 1.f / (float)NT_;
   constexpr bool spectral_in = !(MODE == COL_LOAD || MODE == COL_RAWNL);
+  // pointwise modes keep the FFT's strided ownership (lane ln holds samples
+  // 2(ln + LPC e) and 2(ln + LPC e) + 1); only the t-stencil modes transpose to
+  // contiguous samples
+  constexpr bool STRIDED = !(MODE == COL_NL || MODE == COL_RAWNL);
+  auto tsamp = [&](int i) -> int {
+    if constexpr (STRIDED) return 2 * (ln + LPC * (i >> 1)) + (i & 1);
+    else return t0 + i;
+  };
+  // store this lane's samples u[0..S) of a real column starting at dst
+  auto store_real = [&](float* dst, const float* u) {
+    if constexpr (STRIDED) {
+      float2* d2 = reinterpret_cast<float2*>(dst);
+#pragma unroll
+      for (int e = 0; e < E; ++e) d2[ln + LPC * e] = make_float2(u[2 * e], u[2 * e + 1]);
+    } else {
+      float4* d4 = reinterpret_cast<float4*>(dst + t0);
+#pragma unroll
+      for (int q = 0; q < S / 4; ++q) d4[q] = make_float4(u[4 * q], u[4 * q + 1], u[4 * q + 2], u[4 * q + 3]);
+    }
+  };
   constexpr bool FIN = (MODE == COL_FINAL || MODE == COL_FINAL_SRC);
   constexpr bool rewrite = (MODE == COL_NL || MODE == COL_TAPER || MODE == COL_LOAD || MODE == COL_FINAL_SRC);
   const int row = FIN ? *d.row : 0;
@@ -160,7 +180,14 @@ __global__ __launch_bounds__(256) void k_column(float2* __restrict__ spec, Table
     float u[S];
 
     // ---------------- stage 1: real samples t0 .. t0+S-1 of the column ----------------
-    if constexpr (!spectral_in) {
+    if constexpr (!spectral_in && STRIDED) {
+      const float2* src = reinterpret_cast<const float2*>(d.rin + (long)c * NT_);
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const float2 x = live ? src[ln + LPC * e] : make_float2(0.f, 0.f);
+        u[2 * e] = x.x; u[2 * e + 1] = x.y;
+      }
+    } else if constexpr (!spectral_in) {
       const float4* src = reinterpret_cast<const float4*>(d.rin + (long)c * NT_ + t0);
 #pragma unroll
       for (int q = 0; q < S / 4; ++q) {
@@ -194,24 +221,29 @@ __global__ __launch_bounds__(256) void k_column(float2* __restrict__ spec, Table
       }
       wave_sync();
       cfft<M, true>(v, cb, twi, ln);
-      // transpose to contiguous ownership: lane ln gets z[E ln .. E ln + E)
+      if constexpr (STRIDED) {
 #pragma unroll
-      for (int e = 0; e < E; ++e) cb[pidx(ln + LPC * e)] = v[e];
-      wave_sync();
+        for (int e = 0; e < E; ++e) { u[2 * e] = v[e].x; u[2 * e + 1] = v[e].y; }
+      } else {
+        // transpose to contiguous ownership: lane ln gets z[E ln .. E ln + E)
 #pragma unroll
-      for (int q = 0; q < E; ++q) {
-        const float2 z = cb[pidx(E * ln + q)];
-        u[2 * q] = z.x;
-        u[2 * q + 1] = z.y;
+        for (int e = 0; e < E; ++e) cb[pidx(ln + LPC * e)] = v[e];
+        wave_sync();
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+          const float2 z = cb[pidx(E * ln + q)];
+          u[2 * q] = z.x;
+          u[2 * q + 1] = z.y;
+        }
+        wave_sync();
       }
-      wave_sync();
     }
 
     // ---------------- stage 2: real-space work ----------------
     if constexpr (MODE == COL_NL || MODE == COL_TAPER || MODE == COL_RAWNL) {
       if (d.taper) {
 #pragma unroll
-        for (int i = 0; i < S; ++i) u[i] *= tt[t0 + i];
+        for (int i = 0; i < S; ++i) u[i] *= tt[tsamp(i)];
       }
       if constexpr (NLK != NL_NONE && MODE != COL_TAPER) {
         int nsub = 1;
@@ -297,17 +329,13 @@ __global__ __launch_bounds__(256) void k_column(float2* __restrict__ spec, Table
           }
         }
       }
-      if (MODE == COL_RAWNL && live) {
-        float4* dst = reinterpret_cast<float4*>(d.rout + (long)c * NT_ + t0);
-#pragma unroll
-        for (int q = 0; q < S / 4; ++q) dst[q] = make_float4(u[4 * q], u[4 * q + 1], u[4 * q + 2], u[4 * q + 3]);
-      }
+      if (MODE == COL_RAWNL && live) store_real(d.rout + (long)c * NT_, u);
     } else if constexpr (FIN) {
       if constexpr (MODE == COL_FINAL_SRC) {
         for (int k = 0; k < d.n_src; ++k) {
-          const float* sp = d.src[k] + (long)c * NT_ + t0;
+          const float* sp = d.src[k] + (long)c * NT_;
 #pragma unroll
-          for (int i = 0; i < S; ++i) u[i] += live ? sp[i] : 0.f;
+          for (int i = 0; i < S; ++i) u[i] += live ? sp[tsamp(i)] : 0.f;
         }
         if (d.smask && live && d.smask[c]) {
           const double dl = d.sdelay[c];
@@ -315,7 +343,7 @@ __global__ __launch_bounds__(256) void k_column(float2* __restrict__ spec, Table
           const double cexp = 2.0 * d.s_dur;
 #pragma unroll
           for (int i = 0; i < S; ++i) {
-            const double tm = d.taxis[t0 + i] - dl;
+            const double tm = d.taxis[tsamp(i)] - dl;
             const double env = exp(-pow(1.05 * tm * w0 / (d.s_ncyc * 3.14159265358979323846), cexp));
             u[i] += (float)(env * sin(w0 * tm) * d.s_p0);
           }
@@ -344,17 +372,10 @@ __global__ __launch_bounds__(256) void k_column(float2* __restrict__ spec, Table
           d.pIloss[o] = (float)fmax(0.0, ib - e2);
           d.ibefore[c] = e2;
         }
-        if (c == d.cax) {
-#pragma unroll
-          for (int i = 0; i < S; ++i) d.pax[(long)row * NT_ + t0 + i] = u[i];
-        }
+        if (c == d.cax) store_real(d.pax + (long)row * NT_, u);
       }
       if (live) wmax = fmaxf(wmax, fmaxf(mx, -mn));
-      if (d.rout && live) {
-        float4* dst = reinterpret_cast<float4*>(d.rout + (long)c * NT_ + t0);
-#pragma unroll
-        for (int q = 0; q < S / 4; ++q) dst[q] = make_float4(u[4 * q], u[4 * q + 1], u[4 * q + 2], u[4 * q + 3]);
-      }
+      if (d.rout && live) store_real(d.rout + (long)c * NT_, u);
     } else if constexpr (MODE == COL_LOAD) {
       float mx = 0.f;
       double e2 = 0.0;
@@ -370,22 +391,23 @@ __global__ __launch_bounds__(256) void k_column(float2* __restrict__ spec, Table
       if (live && ln == 0 && d.ibefore) d.ibefore[c] = e2;
       if (live) wmax = fmaxf(wmax, mx);
     } else if constexpr (MODE == COL_STORE) {
-      if (live) {
-        float4* dst = reinterpret_cast<float4*>(d.rout + (long)c * NT_ + t0);
-#pragma unroll
-        for (int q = 0; q < S / 4; ++q) dst[q] = make_float4(u[4 * q], u[4 * q + 1], u[4 * q + 2], u[4 * q + 3]);
-      }
+      if (live) store_real(d.rout + (long)c * NT_, u);
     }
 
     // ---------------- stage 3: rfft back into the spectrum ----------------
     if constexpr (rewrite) {
       float2 v[E];
+      if constexpr (STRIDED) {
 #pragma unroll
-      for (int q = 0; q < E; ++q) cb[pidx(E * ln + q)] = make_float2(u[2 * q], u[2 * q + 1]);
-      wave_sync();
+        for (int e = 0; e < E; ++e) v[e] = make_float2(u[2 * e], u[2 * e + 1]);
+      } else {
 #pragma unroll
-      for (int e = 0; e < E; ++e) v[e] = cb[pidx(ln + LPC * e)];
-      wave_sync();
+        for (int q = 0; q < E; ++q) cb[pidx(E * ln + q)] = make_float2(u[2 * q], u[2 * q + 1]);
+        wave_sync();
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = cb[pidx(ln + LPC * e)];
+        wave_sync();
+      }
       cfft<M, false>(v, cb, twf, ln);
 #pragma unroll
       for (int e = 0; e < E; ++e) cb[pidx(ln + LPC * e)] = v[e];
