@@ -95,8 +95,8 @@ struct ash_plan {
   int64_t ws_bytes = 0;
   bool loaded = false;
   int host_row = -1;
-  int sa_band = 0;
-  int precise_ops = 0;  // evaluate the k-space operators in FP64 (set from ASH_PRECISE_OPS)  // last row index (from each edge) with a nonzero super-absorbing y weight
+  int sa_band = 0;      // last row index (from each edge) with a nonzero super-absorbing y weight
+  int precise_ops = 0;  // evaluate the k-space operators in FP64 (set from ASH_PRECISE_OPS)
   // graph of a plain step
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -364,10 +364,9 @@ void launch_xfwd_L(ash_plan* p, hipStream_t s) {
 }
 template <int L>
 void launch_ypass_L(ash_plan* p, const OpDesc& op, hipStream_t s) {
-  if (op.precise)
-    hipLaunchKernelGGL((k_ypass<L, true>), dim3(p->g.nX * p->g.nfb), dim3(LNT), 0, s, p->spec, p->T, p->g, op);
-  else
-    hipLaunchKernelGGL((k_ypass<L, false>), dim3(p->g.nX * p->g.nfb), dim3(LNT), 0, s, p->spec, p->T, p->g, op);
+  const dim3 grid((p->g.nX / 2) * p->g.nfb);  // one block per (x row pair, f group)
+  if (op.precise) hipLaunchKernelGGL((k_ypass<L, true>), grid, dim3(LNT), 0, s, p->spec, p->T, p->g, op);
+  else hipLaunchKernelGGL((k_ypass<L, false>), grid, dim3(LNT), 0, s, p->spec, p->T, p->g, op);
 }
 template <int L, int EPI>
 void launch_xinv_L(ash_plan* p, const EpiDesc& e, hipStream_t s) {

@@ -223,12 +223,12 @@ __device__ __forceinline__ float2 visit_operator(const OpDesc& op, double kk, co
   if (op.use_prop) {
     if (!lc.kept || kk > op.k_max2) return make_float2(0.f, 0.f);
     if (kk > op.k_start2) {
-      const float pos = (sqrtf(kkf) - op.k_startf) * op.inv_band;
+      const float pos = (__builtin_amdgcn_sqrtf(kkf) - op.k_startf) * op.inv_band;
       amp *= 0.5f * (1.f + cospif(pos));
     }
     const float d2 = (float)(lc.kt2 - kk);
     if (d2 > 0.f) {
-      ph = op.dzp * (kkf / (lc.kt + sqrtf(d2)));
+      ph = op.dzp * kkf * __builtin_amdgcn_rcpf(lc.kt + __builtin_amdgcn_sqrtf(d2));
     } else {
       ph = op.dzp * lc.kt;
       lg = -op.dzp * sqrtf(-d2);
@@ -240,7 +240,7 @@ __device__ __forceinline__ float2 visit_operator(const OpDesc& op, double kk, co
     if (op.obliquity) {
       const float kz2 = (float)(lc.ka2 - kk);
       if (kz2 > 0.f) {
-        const float obl = lc.ka / fmaxf(sqrtf(kz2), lc.ka * 1e-6f);
+        const float obl = lc.ka * __builtin_amdgcn_rsqf(fmaxf(kz2, lc.ka * lc.ka * 1e-12f));
         re = -lc.al * obl;
         im = -lc.as * obl;
       } else {
@@ -291,22 +291,26 @@ __global__ __launch_bounds__(LNT) void k_xfft_fwd(float2* __restrict__ spec, Tab
 // y-pass: forward FFT along y, multiply by the visit operator, inverse FFT.
 // Tile = (kx, f-block).  The 1/(nX nY) normalisation is in the operator.
 // ---------------------------------------------------------------------------
+// One block handles a pair of x rows.  The centred wavenumber grid
+// (ASS:403-408) has half-integer offsets, so natural bins b and n-1-b carry
+// wavenumbers of equal magnitude: rows x and nX-1-x see the same kx^2 and
+// positions y and nY-1-y the same ky^2.  The operator depends on (kx, ky)
+// only through kk = kx^2 + ky^2, so one block evaluates it once per
+// (|ky| index, f bin) into an LDS table and applies it to both rows: a
+// quarter of one evaluation per element.
 template <int L, bool PRECISE>
 __global__ __launch_bounds__(LNT) void k_ypass(float2* __restrict__ spec, Tables T, Geom g, OpDesc op) {
+  constexpr int H = L / 2;
   __shared__ float2 tw[L];
   __shared__ float2 buf[L * LF];
-  __shared__ double sky2[L];
+  __shared__ float2 opt[H * LF];  // [i * LF + line]: operator at natural y bins i and L-1-i
   __shared__ LineConst slc[LF];
   const int tid = threadIdx.x;
-  const int x = blockIdx.x / g.nfb, fb = blockIdx.x % g.nfb;
+  const int ip = blockIdx.x / g.nfb, fb = blockIdx.x % g.nfb;
   const int f0 = fb * LF;
-  float2* base = spec + (long)x * g.nY * g.MP + f0;
   const long sy = g.MP;
   const int nvalid = min(LF, g.M + 1 - f0);
   load_tw<L>(tw, T.twy, tid, LNT);
-  const double kx = T.kx[x];
-  const double kx2 = kx * kx;
-  for (int i = tid; i < L; i += LNT) sky2[i] = kx2 + T.ky[i] * T.ky[i];
   if (tid < LF) {
     const int m = min(f0 + tid, g.MP - 1);
     const double kt = T.kt[m], ka = T.ka[m];
@@ -322,28 +326,49 @@ __global__ __launch_bounds__(LNT) void k_ypass(float2* __restrict__ spec, Tables
     slc[tid] = lc;
   }
   __syncthreads();
-  using LY = Lay<L, LF, 0, true>;
-  auto ld = [&](int line, int pos) -> float2 {
-    return line < nvalid ? base[pos * sy + line] : make_float2(0.f, 0.f);
-  };
-  auto st_lds = [&](int line, int pos, float2 v) { buf[LY::idx(line, pos)] = v; };
-  fft_run<L, false, LNT, LF, LY>(buf, tw, tid, ld, st_lds);
-  // the inverse pass 0 reads back from LDS through the operator
-  // with the line-fast butterfly mapping and LNT a multiple of LF, every
-  // butterfly of this thread is on line tid % LF: keep its constants in registers
   static_assert(LNT % LF == 0, "the line of a thread must be fixed");
   const int myline = tid % LF;
   const LineConst lc = slc[myline];
-  auto ld_op = [&](int line, int pos) -> float2 {
-    const float2 v = buf[LY::idx(line, pos)];
-    if constexpr (PRECISE) return cmul(v, visit_operator_fp64(op, sky2[pos], lc, f0 + myline));
-    else return cmul(v, visit_operator(op, sky2[pos], lc, f0 + myline));
-  };
-  auto st = [&](int line, int pos, float2 v) {
-    if (line < nvalid) base[pos * sy + line] = v;
-  };
-  __syncthreads();
-  fft_run<L, true, LNT, LF, LY>(buf, tw, tid, ld_op, st);
+  {
+    const double kx = T.kx[ip];
+    const double kx2 = kx * kx;
+    // entry e = tid + LNT k is on line e % LF = myline
+#pragma unroll 1
+    for (int e = tid; e < H * LF; e += LNT) {
+      const double ky = T.ky[e / LF];
+      const double kk = kx2 + ky * ky;
+      if constexpr (PRECISE) opt[e] = visit_operator_fp64(op, kk, lc, f0 + myline);
+      else opt[e] = visit_operator(op, kk, lc, f0 + myline);
+    }
+  }
+  // the table is read after the barriers of the forward transform
+  using LY = Lay<L, LF, 0, true>;
+#pragma unroll 1
+  for (int h = 0; h < 2; ++h) {
+    // an opaque copy of the thread index per row: without it the compiler
+    // keeps every thread-dependent LDS address and twiddle of both transforms
+    // live across the rows (~70 more VGPRs, one block fewer per CU)
+    int t = tid;
+    asm volatile("" : "+v"(t));
+    const int x = h == 0 ? ip : g.nX - 1 - ip;
+    float2* base = spec + (long)x * g.nY * g.MP + f0;
+    auto ld = [&](int line, int pos) -> float2 {
+      return line < nvalid ? base[pos * sy + line] : make_float2(0.f, 0.f);
+    };
+    auto st_lds = [&](int line, int pos, float2 v) { buf[LY::idx(line, pos)] = v; };
+    // pass 0 stores to LDS only after its barrier, which follows every read of
+    // the previous row's last pass
+    fft_run<L, false, LNT, LF, LY>(buf, tw, t, ld, st_lds);
+    auto ld_op = [&](int line, int pos) -> float2 {
+      const int i = pos < H ? pos : L - 1 - pos;
+      return cmul(buf[LY::idx(line, pos)], opt[i * LF + line]);
+    };
+    auto st = [&](int line, int pos, float2 v) {
+      if (line < nvalid) base[pos * sy + line] = v;
+    };
+    __syncthreads();
+    fft_run<L, true, LNT, LF, LY>(buf, tw, t, ld_op, st);
+  }
 }
 
 // ---------------------------------------------------------------------------
