@@ -329,6 +329,7 @@ __global__ __launch_bounds__(LNT) void k_ypass(float2* __restrict__ spec, Tables
   static_assert(LNT % LF == 0, "the line of a thread must be fixed");
   const int myline = tid % LF;
   const LineConst lc = slc[myline];
+  using LY = Lay<L, LF, 0, true>;
   {
     const double kx = T.kx[ip];
     const double kx2 = kx * kx;
@@ -342,7 +343,6 @@ __global__ __launch_bounds__(LNT) void k_ypass(float2* __restrict__ spec, Tables
     }
   }
   // the table is read after the barriers of the forward transform
-  using LY = Lay<L, LF, 0, true>;
 #pragma unroll 1
   for (int h = 0; h < 2; ++h) {
     // an opaque copy of the thread index per row: without it the compiler
@@ -415,39 +415,51 @@ __global__ __launch_bounds__(LNT) void k_xfft_inv(float2* __restrict__ spec, Tab
     };
     fft_run<L, true, LNT, LF, LY>(buf, tw, tid, ld, st);
   } else {
-    auto st_lds = [&](int line, int pos, float2 v) { buf[LY::idx(line, pos)] = v; };
+    // the inverse transform's last pass stores into LDS with the scale, the
+    // DC/Nyquist projection and the frequency-weighted mask already applied,
+    // so the SA stencil below reads masked neighbours without recomputing them
+    // x tables in LDS while that keeps the blocks per CU (L <= 256), else read
+    // through the cache
+    constexpr bool TLDS = L <= 256;
+    __shared__ float slgx[TLDS ? L : 1], swx[TLDS ? L : 1];
+    __shared__ float sfwe[LF];
+    if constexpr (TLDS) {
+      for (int i = tid; i < L; i += LNT) {
+        slgx[i] = T.lgx[i];
+        swx[i] = T.sa_wx[i];
+      }
+    }
+    const float* lgx = TLDS ? slgx : T.lgx;
+    const float* wx = TLDS ? swx : T.sa_wx;
+    if (tid < LF) sfwe[tid] = T.fw_exp[min(f0 + tid, g.MP - 1)];
+    const float lgy = T.lgy[y];
+    // (the barrier of the transform's first pass orders these stores before the reads)
+    auto st_lds = [&](int line, int pos, float2 v) {
+      const int m = f0 + line;
+      v = cscale(v, e.scale);
+      if (m == 0 || m == g.M) v.y = 0.f;  // irfft/rfft round trip of the reference
+      if (e.fw) v = cscale(v, exp2f(sfwe[line] * (lgx[pos] + lgy)));
+      buf[LY::idx(line, pos)] = v;
+    };
     fft_run<L, true, LNT, LF, LY>(buf, tw, tid, ld, st_lds);
     __syncthreads();
-    const float lgy = T.lgy[y];
-    // element e -> (line, pos), line fastest: coalesced 128-byte rows
+    // element e -> (line, pos), line fastest: coalesced 128-byte rows; the
+    // line of a thread is fixed (LNT is a multiple of LF)
     constexpr int PER = (L * LF) / LNT;
+    const int line = tid % LF;
+    const float cw = T.sa_cw[min(f0 + line, g.MP - 1)];
     float2 res[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
-      const int el = tid + q * LNT;
-      const int line = el % LF, pos = el / LF;
-      const int m = f0 + line;
-      float2 v = cscale(buf[LY::idx(line, pos)], e.scale);
-      if (m == 0 || m == g.M) v.y = 0.f;  // irfft/rfft round trip of the reference
-      float fwm = 1.f;
-      if (e.fw) fwm = exp2f(T.fw_exp[min(m, g.MP - 1)] * (T.lgx[pos] + lgy));
-      v = cscale(v, fwm);
+      const int pos = (tid + q * LNT) / LF;
+      float2 v = buf[LY::idx(line, pos)];
       if (e.sa) {
-        const float w = T.sa_wx[pos];
+        const float w = wx[pos];
         if (w != 0.f) {
-          // neighbours, with the same projection and mask applied
-          auto nb = [&](int p) -> float2 {
-            float2 u = cscale(buf[LY::idx(line, p)], e.scale);
-            if (m == 0 || m == g.M) u.y = 0.f;
-            float f = 1.f;
-            if (e.fw) f = exp2f(T.fw_exp[min(m, g.MP - 1)] * (T.lgx[p] + lgy));
-            return cscale(u, f);
-          };
           float2 d;
-          if (pos == 0) d = cscale(csub(nb(1), v), e.invdx);
-          else if (pos == L - 1) d = cscale(csub(v, nb(L - 2)), e.invdx);
-          else d = cscale(csub(nb(pos + 1), nb(pos - 1)), e.inv2dx);
-          const float cw = T.sa_cw[min(m, g.MP - 1)];
+          if (pos == 0) d = cscale(csub(buf[LY::idx(line, 1)], v), e.invdx);
+          else if (pos == L - 1) d = cscale(csub(v, buf[LY::idx(line, L - 2)]), e.invdx);
+          else d = cscale(csub(buf[LY::idx(line, pos + 1)], buf[LY::idx(line, pos - 1)]), e.inv2dx);
           // coeff * dF with coeff = -i c0/omega
           const float2 cd = make_float2(cw * d.y, -cw * d.x);
           const float2 pin = pos < L / 2 ? cscale(cadd(v, cd), 0.5f) : cscale(csub(v, cd), 0.5f);
@@ -456,11 +468,9 @@ __global__ __launch_bounds__(LNT) void k_xfft_inv(float2* __restrict__ spec, Tab
       }
       res[q] = v;
     }
+    if (line < nvalid) {
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      const int el = tid + q * LNT;
-      const int line = el % LF, pos = el / LF;
-      if (line < nvalid) base[pos * sx + line] = res[q];
+      for (int q = 0; q < PER; ++q) base[((tid + q * LNT) / LF) * sx + line] = res[q];
     }
   }
 }
