@@ -616,22 +616,24 @@ __device__ __forceinline__ float slope(float um, float u0, float up) {
   if (MC) return minmod3f(2.f * dm, 0.5f * (dm + dp), 2.f * dp);
   return minmod2f(dm, dp);
 }
-// KT central-upwind flux at an interface (ASS:771-793).  When both local
-// speeds -u have one sign the formula reduces exactly to the upwind flux
-// (a- = 0 gives f-, a+ = 0 gives f+); only a sign change across the interface
-// needs the quotient, evaluated with the fast reciprocal.
+// KT central-upwind flux at an interface (ASS:771-793),
+//   (a+ f- - a- f+ - a+ a- (u+ - u-)) / (a+ - a-),  f(u) = -u^2/2,
+//   a+ = max(-u-, -u+, 0), a- = min(-u-, -u+, 0).
+// For Burgers the quotient has closed forms: with both speeds of one sign it
+// is the upwind flux (f- or f+); with u- > 0 > u+ it is 1.5 u- u+; with
+// u- < 0 < u+ it is -(u-^2 + 3 u- u+ + u+^2)/2.  The Lax-Friedrichs fallback
+// for a+ - a- < 1e-14 (ASS:792-793) is kept.
 __device__ __forceinline__ float kt_iface(float umi, float upi) {
   const float ap = fmaxf(fmaxf(-umi, -upi), 0.f);
   const float am = fminf(fminf(-umi, -upi), 0.f);
   const float fm = -0.5f * umi * umi;
   const float fp = -0.5f * upi * upi;
-  const float den = ap - am;
-  if (den < 1e-14f) {
+  if (ap - am < 1e-14f) {
     return 0.5f * (fm + fp) - 0.5f * fmaxf(fabsf(umi), fabsf(upi)) * (upi - umi);
   }
   if (am == 0.f) return fm;
   if (ap == 0.f) return fp;
-  return __fdividef(ap * fm - am * fp - ap * am * (upi - umi), den);
+  return umi > 0.f ? 1.5f * umi * upi : -0.5f * fmaf(umi, umi + 3.f * upi, upi * upi);
 }
 
 // Semi-discrete KT rhs for S samples t0..t0+S-1 from u[-2..S+1] (index offset 2).
