@@ -76,10 +76,21 @@ _PROTOS = {
     "ash_profile_step": ([c_vp, ctypes.POINTER(AshStepIO), c_i32, ctypes.POINTER(ctypes.c_float),
                           ctypes.POINTER(c_i32), c_vp], c_i32),
     "ash_plan_workspace_bytes": ([c_vp], ctypes.c_int64),
+    "ash_net_param_count": ([], ctypes.c_int64),
+    "ash_net_create": ([c_i32, c_i32, ctypes.POINTER(c_vp)], c_i32),
+    "ash_net_destroy": ([c_vp], None),
+    "ash_net_set_params": ([c_vp, c_vp, c_vp], c_i32),
+    "ash_net_get_params": ([c_vp, c_vp, c_vp], c_i32),
+    "ash_net_forward": ([c_vp, c_vp, c_i32, c_vp, c_vp, c_vp], c_i32),
+    "ash_net_loss_grad": ([c_vp, c_vp, c_vp, c_i32, c_dbl, c_vp, c_vp, c_vp], c_i32),
+    "ash_net_adam": ([c_vp, c_vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, c_i32, c_vp],
+                     c_i32),
+    "ash_net_apply": ([c_vp, c_vp, c_i32, c_dbl, c_dbl, c_dbl, c_vp, c_vp], c_i32),
+    "ash_plan_set_learned": ([c_vp, c_vp, c_dbl], c_i32),
 }
 
 KERNEL_KINDS = ("xfft_fwd", "ypass", "xfft_inv", "ysa", "column_nl", "column_taper", "column_final",
-                "column_other")
+                "column_other", "net")
 
 EXPORTED = tuple(_PROTOS)
 

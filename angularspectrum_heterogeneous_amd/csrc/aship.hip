@@ -12,6 +12,7 @@
 
 #include "../../include/aship.h"
 #include "kernels.h"
+#include "learned.h"
 
 using namespace ash;
 
@@ -109,6 +110,26 @@ struct ash_plan {
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_ev;
   std::vector<hipEvent_t> ev_pool;
   size_t ev_next = 0;
+  // learned nonlinear step (ash_plan_set_learned)
+  ash_net* net = nullptr;
+  double steep_max = 0.0;
+  float* rscratch = nullptr;  // real field between the column passes [nX*nY*nT]
+  int32_t* nflag = nullptr;   // 1 where a column took the KT update [nX*nY]
+};
+
+struct ash_net {
+  int nT = 0, max_cols = 0, n_cu = 0;
+  float* params = nullptr;  // float32 master parameters
+  float* m = nullptr;       // Adam moments
+  float* v = nullptr;
+  float* mask = nullptr;    // 1 on trainable entries
+  uint16_t* Wm = nullptr;   // bf16 [layer][k][co][ci]
+  uint16_t* Wt = nullptr;   // bf16 [layer][k][ci][co], flipped in k
+  uint16_t* X[4] = {};      // training activations [max_cols][nT][32]
+  uint16_t* Dg[2] = {};     // back-propagated gradients, same shape
+  float* y = nullptr;       // [max_cols][nT]
+  float* amp = nullptr;     // [max_cols]
+  std::vector<void*> allocs;
 };
 
 namespace {
@@ -406,6 +427,7 @@ void launch_col_M(ash_plan* p, const ColDesc& cd, hipStream_t s) {
       break;
     case COL_STORE: launch_col_MN<M, COL_STORE, NL_NONE>(p, cd, s); break;
     case COL_RAWNL: launch_col_MM<M, COL_RAWNL>(p, cd, s); break;
+    case COL_TOREAL: launch_col_MN<M, COL_TOREAL, NL_NONE>(p, cd, s); break;
     default: break;
   }
 }
@@ -482,6 +504,47 @@ EpiDesc epi_base(const ash_plan* p) {
   return e;
 }
 
+int net_apply(ash_net* n, float* u, int ncols, double c, double steep_max, double f0_dT, int32_t* flag,
+              hipStream_t s) {
+  const int grid = std::min(ncols, std::max(n->n_cu, 1));  // one 127-131 KB LDS block per CU
+  const float inv = (float)(1.0 / (2.0 * M_PI * f0_dT));
+  switch (n->nT) {
+    case 256: hipLaunchKernelGGL(net::k_net_infer<256>, dim3(grid), dim3(net::INF_THREADS), 0, s, u, ncols, n->Wm, n->params, (float)c, (float)steep_max, inv, flag); break;
+    case 512: hipLaunchKernelGGL(net::k_net_infer<512>, dim3(grid), dim3(net::INF_THREADS), 0, s, u, ncols, n->Wm, n->params, (float)c, (float)steep_max, inv, flag); break;
+    case 1024: hipLaunchKernelGGL(net::k_net_infer<1024>, dim3(grid), dim3(net::INF_THREADS), 0, s, u, ncols, n->Wm, n->params, (float)c, (float)steep_max, inv, flag); break;
+    case 2048: hipLaunchKernelGGL(net::k_net_infer<2048>, dim3(grid), dim3(net::INF_THREADS), 0, s, u, ncols, n->Wm, n->params, (float)c, (float)steep_max, inv, flag); break;
+    default: return fail(ASH_E_UNSUPPORTED, "network column length must be 256, 512, 1024 or 2048");
+  }
+  return ASH_OK;
+}
+
+// Learned nonlinear update in place of the KT column pass: irfft * taper to a
+// real field, network on the smooth columns, KT on the flagged ones, rfft.
+void learned_nl(ash_plan* p, ColDesc cd, double dz, hipStream_t s) {
+  cd.mode = COL_TOREAL;
+  cd.taper = 1;
+  cd.rout = p->rscratch;
+  column(p, cd, s);
+  {
+    ProfScope ps(p, ASH_K_NET, s);
+    net_apply(p->net, p->rscratch, p->g.nX * p->g.nY, dz * p->N / p->d.dT, p->steep_max, p->d.f0 * p->d.dT,
+              p->nflag, s);
+  }
+  cd.mode = COL_RAWNL;
+  cd.taper = 0;
+  cd.rin = p->rscratch;
+  cd.rout = p->rscratch;
+  cd.cflag = p->nflag;
+  column(p, cd, s);
+  cd.mode = COL_LOAD;
+  cd.rin = p->rscratch;
+  cd.rout = nullptr;
+  cd.cflag = nullptr;
+  cd.ibefore = nullptr;
+  cd.stats = nullptr;
+  column(p, cd, s);
+}
+
 // The composite march step (ASS:874-913) on the spectral state, leaving the
 // state after the last attenuation visit with epilogue `epi_last` fused.
 void enqueue_composite(ash_plan* p, double dz, int epi_last, const EpiDesc& elast, hipStream_t s) {
@@ -497,9 +560,13 @@ void enqueue_composite(ash_plan* p, double dz, int epi_last, const EpiDesc& elas
     ypass(p, make_op(p, true, dz / 2, true, dz * 0.5, dz), s);
     e.half = 1;
     xinv(p, 1, e, s);
-    cd.mode = COL_NL;
-    cd.taper = 1;
-    column(p, cd, s);
+    cd.taper = 1;  // also the taper of the COL_TAPER pass below
+    if (p->net) {
+      learned_nl(p, cd, dz, s);
+    } else {
+      cd.mode = COL_NL;
+      column(p, cd, s);
+    }
     xfwd(p, s);
     ypass(p, make_op(p, true, dz / 2, false, 0.0, dz), s);
     xinv(p, 1, e, s);
@@ -872,5 +939,175 @@ int ash_profile_step(ash_plan* p, const ash_step_io* io, int32_t n_iter, float* 
 }
 
 int64_t ash_plan_workspace_bytes(const ash_plan* p) { return p ? p->ws_bytes : 0; }
+
+// ---------------------------------------------------------------------------
+// Learned nonlinear step
+// ---------------------------------------------------------------------------
+int64_t ash_net_param_count(void) { return net::PARAM_COUNT; }
+
+int ash_net_create(int32_t nT, int32_t max_cols, ash_net** out) {
+  if (!out) return fail(ASH_E_ARG, "null argument");
+  *out = nullptr;
+  if (!(nT == 256 || nT == 512 || nT == 1024 || nT == 2048))
+    return fail(ASH_E_UNSUPPORTED, "network column length must be 256, 512, 1024 or 2048");
+  if (max_cols < 1) return fail(ASH_E_ARG, "max_cols must be positive");
+  ash_net* n = new ash_net();
+  n->nT = nT;
+  n->max_cols = max_cols;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&n->n_cu, hipDeviceAttributeMultiprocessorCount, dev);
+  auto alloc = [&](void** ptr, size_t bytes) -> int {
+    hipError_t e = hipMalloc(ptr, bytes);
+    if (e != hipSuccess) return fail(ASH_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    n->allocs.push_back(*ptr);
+    e = hipMemset(*ptr, 0, bytes);
+    if (e != hipSuccess) return fail(ASH_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
+    return ASH_OK;
+  };
+  const size_t np = net::PARAM_COUNT, act = (size_t)max_cols * nT * net::C;
+  int rc = alloc((void**)&n->params, np * 4);
+  if (!rc) rc = alloc((void**)&n->m, np * 4);
+  if (!rc) rc = alloc((void**)&n->v, np * 4);
+  if (!rc) rc = alloc((void**)&n->mask, np * 4);
+  if (!rc) rc = alloc((void**)&n->Wm, (size_t)net::NLAY * net::W_SIZE * 2);
+  if (!rc) rc = alloc((void**)&n->Wt, (size_t)net::NLAY * net::W_SIZE * 2);
+  for (int j = 0; j < 4 && !rc; ++j) rc = alloc((void**)&n->X[j], act * 2);
+  for (int j = 0; j < 2 && !rc; ++j) rc = alloc((void**)&n->Dg[j], act * 2);
+  if (!rc) rc = alloc((void**)&n->y, (size_t)max_cols * nT * 4);
+  if (!rc) rc = alloc((void**)&n->amp, (size_t)max_cols * 4);
+  if (!rc) {
+    // trainable entries: W1 input channels 0-1, W4 and b4 output channel 0, all of layers 2-3
+    std::vector<float> mk(np, 1.f);
+    for (int co = 0; co < net::C; ++co)
+      for (int ci = 2; ci < net::C; ++ci)
+        for (int k = 0; k < net::KW; ++k) mk[(co * net::C + ci) * net::KW + k] = 0.f;
+    const size_t o4 = 3 * (size_t)net::LAYER_SIZE;
+    for (int co = 1; co < net::C; ++co) {
+      for (int i = 0; i < net::C * net::KW; ++i) mk[o4 + co * net::C * net::KW + i] = 0.f;
+      mk[o4 + net::W_SIZE + co] = 0.f;
+    }
+    hipError_t e = hipMemcpy(n->mask, mk.data(), np * 4, hipMemcpyHostToDevice);
+    if (e != hipSuccess) rc = fail(ASH_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+  }
+  if (rc) { ash_net_destroy(n); return rc; }
+  *out = n;
+  return ASH_OK;
+}
+
+void ash_net_destroy(ash_net* n) {
+  if (!n) return;
+  for (void* a : n->allocs) (void)hipFree(a);
+  delete n;
+}
+
+int ash_net_set_params(ash_net* n, const float* params, void* stream) {
+  if (!n || !params) return fail(ASH_E_ARG, "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  HIP_TRY(hipMemcpyAsync(n->params, params, net::PARAM_COUNT * 4, hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(net::k_net_pack, dim3((net::NLAY * net::W_SIZE + 255) / 256), dim3(256), 0, s, n->params, n->Wm,
+                     n->Wt);
+  return check_launch();
+}
+
+int ash_net_get_params(ash_net* n, float* params, void* stream) {
+  if (!n || !params) return fail(ASH_E_ARG, "null argument");
+  HIP_TRY(hipMemcpyAsync(params, n->params, net::PARAM_COUNT * 4, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  return ASH_OK;
+}
+
+namespace {
+int net_forward(ash_net* n, const float* u, int ncols, float* y, float* amp, hipStream_t s) {
+  const int nT = n->nT;
+  hipLaunchKernelGGL(net::k_net_feat, dim3(ncols), dim3(256), 0, s, u, nT, n->X[0], amp);
+  const dim3 grid(nT / net::TT, ncols);
+  for (int j = 0; j < 3; ++j)
+    hipLaunchKernelGGL(net::k_net_conv<0>, grid, dim3(256), 0, s, n->X[j], nT, n->Wm + j * net::W_SIZE,
+                       n->params + j * net::LAYER_SIZE + net::W_SIZE, (const uint16_t*)nullptr, n->X[j + 1],
+                       (float*)nullptr);
+  hipLaunchKernelGGL(net::k_net_conv<1>, grid, dim3(256), 0, s, n->X[3], nT, n->Wm + 3 * net::W_SIZE,
+                     n->params + 3 * net::LAYER_SIZE + net::W_SIZE, (const uint16_t*)nullptr, (uint16_t*)nullptr, y);
+  return check_launch();
+}
+}  // namespace
+
+int ash_net_forward(ash_net* n, const float* u, int32_t ncols, float* y, float* amp, void* stream) {
+  if (!n || !u || !y || !amp) return fail(ASH_E_ARG, "null argument");
+  if (ncols < 1 || ncols > n->max_cols) return fail(ASH_E_ARG, "ncols out of range");
+  return net_forward(n, u, ncols, y, amp, (hipStream_t)stream);
+}
+
+int ash_net_loss_grad(ash_net* n, const float* u, const float* un, int32_t ncols, double c, double* loss,
+                      float* grads, void* stream) {
+  if (!n || !u || !un || !loss || !grads) return fail(ASH_E_ARG, "null argument");
+  if (ncols < 1 || ncols > n->max_cols) return fail(ASH_E_ARG, "ncols out of range");
+  if (!(c > 0)) return fail(ASH_E_ARG, "c must be positive");
+  hipStream_t s = (hipStream_t)stream;
+  const int nT = n->nT;
+  int rc = net_forward(n, u, ncols, n->y, n->amp, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(net::k_net_loss, dim3((nT + 255) / 256, ncols), dim3(256), 0, s, n->y, u, un, n->amp, nT,
+                     (float)c, (float)(1.0 / ((double)ncols * nT)), n->Dg[0], loss);
+  HIP_TRY(hipMemsetAsync(grads, 0, net::PARAM_COUNT * 4, s));
+  const dim3 grid(nT / net::TT, ncols);
+  int cur = 0;
+  for (int j = 3; j >= 0; --j) {
+    float* gW = grads + j * net::LAYER_SIZE;
+    hipLaunchKernelGGL(net::k_net_wgrad, grid, dim3(256), 0, s, n->Dg[cur], n->X[j], nT, gW, gW + net::W_SIZE);
+    if (j > 0) {
+      hipLaunchKernelGGL(net::k_net_conv<2>, grid, dim3(256), 0, s, n->Dg[cur], nT, n->Wt + j * net::W_SIZE,
+                         (const float*)nullptr, n->X[j], n->Dg[1 - cur], (float*)nullptr);
+      cur = 1 - cur;
+    }
+  }
+  return check_launch();
+}
+
+int ash_net_adam(ash_net* n, const float* grads, float lr, float b1, float b2, float eps, int32_t step,
+                 void* stream) {
+  if (!n || !grads) return fail(ASH_E_ARG, "null argument");
+  if (step < 1) return fail(ASH_E_ARG, "step counts from 1");
+  hipStream_t s = (hipStream_t)stream;
+  const float bc1 = (float)(1.0 - std::pow((double)b1, step)), bc2 = (float)(1.0 - std::pow((double)b2, step));
+  hipLaunchKernelGGL(net::k_net_adam, dim3((net::PARAM_COUNT + 255) / 256), dim3(256), 0, s, n->params, grads, n->m,
+                     n->v, n->mask, net::PARAM_COUNT, lr, b1, b2, eps, bc1, bc2);
+  hipLaunchKernelGGL(net::k_net_pack, dim3((net::NLAY * net::W_SIZE + 255) / 256), dim3(256), 0, s, n->params, n->Wm,
+                     n->Wt);
+  return check_launch();
+}
+
+int ash_net_apply(ash_net* n, float* u, int32_t ncols, double c, double steep_max, double f0_dT, int32_t* flag,
+                  void* stream) {
+  if (!n || !u || !flag) return fail(ASH_E_ARG, "null argument");
+  if (ncols < 1) return fail(ASH_E_ARG, "ncols must be positive");
+  if (!(f0_dT > 0)) return fail(ASH_E_ARG, "f0 dT must be positive");
+  int rc = net_apply(n, u, ncols, c, steep_max, f0_dT, flag, (hipStream_t)stream);
+  if (rc) return rc;
+  return check_launch();
+}
+
+int ash_plan_set_learned(ash_plan* p, ash_net* n, double steep_max) {
+  if (!p) return fail(ASH_E_ARG, "null plan");
+  if (n) {
+    if (!p->d.useSplitStep || p->d.fluxScheme != 1)
+      return fail(ASH_E_UNSUPPORTED, "the learned step replaces the split-step KT update only");
+    if (n->nT != p->d.nT) return fail(ASH_E_ARG, "network column length differs from the plan's nT");
+    const size_t ncol = (size_t)p->d.nX * p->d.nY;
+    if (!p->rscratch) {
+      HIP_TRY(hipMalloc(&p->rscratch, ncol * p->d.nT * sizeof(float)));
+      p->allocs.push_back(p->rscratch);
+      HIP_TRY(hipMalloc(&p->nflag, ncol * sizeof(int32_t)));
+      p->allocs.push_back(p->nflag);
+      HIP_TRY(hipMemset(p->nflag, 0, ncol * sizeof(int32_t)));
+      p->ws_bytes += (int64_t)(ncol * p->d.nT * sizeof(float) + ncol * sizeof(int32_t));
+    }
+  }
+  p->net = n;
+  p->steep_max = steep_max;
+  // the captured step graph holds the other update
+  if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
+  if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
+  return ASH_OK;
+}
 
 }  // extern "C"

@@ -175,7 +175,8 @@ __global__ __launch_bounds__(256) void k_column(float2* __restrict__ spec, Table
 
   for (int cbase = blockIdx.x * per_iter; cbase < ncols; cbase += gridDim.x * per_iter) {
     const int c = cbase + wave * CPW + cw;
-    const bool live = c < ncols;
+    const bool live = c < ncols && (!d.cflag || d.cflag[c] != 0);
+    if (CPW == 1 && !live) continue;  // one column per wave: skip it whole
     float2* sc = spec + (long)c * g.MP;
     float u[S];
 
@@ -391,6 +392,12 @@ __global__ __launch_bounds__(256) void k_column(float2* __restrict__ spec, Table
       if (live && ln == 0 && d.ibefore) d.ibefore[c] = e2;
       if (live) wmax = fmaxf(wmax, mx);
     } else if constexpr (MODE == COL_STORE) {
+      if (live) store_real(d.rout + (long)c * NT_, u);
+    } else if constexpr (MODE == COL_TOREAL) {
+      if (d.taper) {
+#pragma unroll
+        for (int i = 0; i < S; ++i) u[i] *= tt[tsamp(i)];
+      }
       if (live) store_real(d.rout + (long)c * NT_, u);
     }
 

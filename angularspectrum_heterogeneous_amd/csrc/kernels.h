@@ -569,6 +569,7 @@ enum ColMode : int {
   COL_STORE = 4,   // spectrum -> real field
   COL_RAWNL = 5,   // real field in -> nonlinear step -> real field out (no FFT)
   COL_FINAL_SRC = 6,  // COL_FINAL on a step that injects source slices
+  COL_TOREAL = 7,  // spectrum -> taper -> real field (the learned step's input)
 };
 
 enum NLKind : int { NL_NONE = 0, NL_KT = 1, NL_RUSANOV = 2, NL_TVD = 3, NL_KT_MINMOD = 4 };
@@ -583,7 +584,8 @@ struct ColDesc {
   float beta_tvd;
   // real-field io
   const float* rin;  // COL_LOAD / COL_RAWNL input [nX*nY*nT]
-  float* rout;       // COL_STORE / COL_RAWNL / COL_FINAL(optional) output
+  float* rout;       // COL_STORE / COL_RAWNL / COL_TOREAL / COL_FINAL(optional) output
+  const int32_t* cflag;  // when set, only columns with cflag[c] != 0 are processed
   // reductions (COL_FINAL / COL_LOAD)
   const int* row;    // device step counter; outputs go to row `*row`
   int max_rows;

@@ -1,0 +1,346 @@
+// Learned nonlinear column step: a 4-layer 1-D conv net over t that stands in
+// for one dZ of the KT update (ASS:833-839) on smooth columns.
+//
+//   A = max_t |u|, x = u / A, X0[t] = (x, x^2, 0...)          32 channels, bf16
+//   X_j = relu(conv9(X_{j-1}, W_j) + b_j), j = 1..3           32 channels, bf16
+//   y   = conv9(X_3, W_4) + b_4 (channel 0), u_new = u + c A^2 y,  c = dZ N / dT
+//
+// conv9 is the zero-padded cross-correlation out[t] = sum_k W_k in[t + k - 4].
+// Every layer is a 32 x 32 x 9 convolution (layer 1 reads channels 0-1, layer
+// 4 writes channel 0; the other weights are zero), run on the matrix cores as
+// D[co][t] += W_k[co][ci] X[t + k - 4][ci] with v_mfma_f32_16x16x32_bf16:
+// lane l holds A[l%16][8(l/16) + j], B[8(l/16) + j][l%16] and
+// D[4(l/16) + i][l%16].  Activations are [t][32 channels] rows of bfloat16;
+// accumulation is float32.  The weight gradients are float32 VALU sums.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ash {
+namespace net {
+
+constexpr int C = 32;       // channels
+constexpr int KW = 9;       // kernel width
+constexpr int HW = KW / 2;  // half width
+constexpr int NLAY = 4;
+constexpr int W_SIZE = C * C * KW;
+constexpr int LAYER_SIZE = W_SIZE + C;           // W[co][ci][k] then b[co], float32
+constexpr int PARAM_COUNT = NLAY * LAYER_SIZE;
+constexpr int RS = 40;  // LDS row stride in bf16 (80 B: 16-byte aligned, spreads the banks)
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint16_t u16;
+
+__device__ __forceinline__ u16 f2bf(float x) { return __builtin_bit_cast(u16, (__bf16)x); }
+__device__ __forceinline__ float bf2f(u16 h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ uint2 pack4(float a, float b, float c, float d) {
+  return make_uint2((uint32_t)f2bf(a) | ((uint32_t)f2bf(b) << 16), (uint32_t)f2bf(c) | ((uint32_t)f2bf(d) << 16));
+}
+
+// A operands of one layer for this lane: a[mt][k] = W_k[16 mt + l%16][8 (l/16) .. +8).
+// Wl is [KW][C][C] bf16 (LDS or global), 16-byte aligned.
+template <int NMT>
+__device__ __forceinline__ void load_a(const u16* Wl, int lane, bf16x8 (&a)[2][KW]) {
+#pragma unroll
+  for (int mt = 0; mt < NMT; ++mt)
+#pragma unroll
+    for (int k = 0; k < KW; ++k)
+      a[mt][k] = *reinterpret_cast<const bf16x8*>(Wl + (k * C + 16 * mt + (lane & 15)) * C + 8 * (lane >> 4));
+}
+
+// 32 (or 16) output channels x 16 positions p0..p0+15 from LDS rows p0-4 .. p0+19.
+template <int NMT>
+__device__ __forceinline__ void conv_tile(const u16* in, int p0, int lane, const bf16x8 (&a)[2][KW], f32x4 (&acc)[2]) {
+#pragma unroll
+  for (int mt = 0; mt < NMT; ++mt) acc[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int k = 0; k < KW; ++k) {
+    const bf16x8 b = *reinterpret_cast<const bf16x8*>(in + (p0 + (lane & 15) + k - HW) * RS + 8 * (lane >> 4));
+#pragma unroll
+    for (int mt = 0; mt < NMT; ++mt) acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt][k], b, acc[mt], 0, 0, 0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused inference over real columns R[c][t] (in place), persistent blocks.
+// Each column: A = max|u| and the steepness max|u[t+1]-u[t]| / (A w0 dT) in
+// one pass; a column over the threshold is flagged (flag[c] = 1) and left to
+// the KT kernel; otherwise the four layers run on chunks of TC outputs with a
+// 16-sample halo on each side, all activations in LDS.
+// ---------------------------------------------------------------------------
+constexpr int INF_THREADS = 512;
+constexpr int INF_WAVES = INF_THREADS / 64;
+constexpr int TC = 256;
+constexpr int ROWS = TC + 48;
+
+template <int NT_>
+__global__ __launch_bounds__(INF_THREADS) void k_net_infer(float* __restrict__ R, int ncols, const u16* __restrict__ Wm,
+                                                           const float* __restrict__ params, float c, float steep_max,
+                                                           float inv_w0dt, int32_t* __restrict__ flag) {
+  static_assert(NT_ % TC == 0, "column length must be a multiple of the chunk");
+  __shared__ __align__(16) u16 sW[NLAY * KW * C * C];
+  __shared__ __align__(16) u16 buf[2][ROWS * RS];
+  __shared__ float scol[NT_];
+  __shared__ float sbias[NLAY][C];
+  __shared__ float sred[2][INF_WAVES];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  {
+    const uint4* src = reinterpret_cast<const uint4*>(Wm);
+    uint4* dst = reinterpret_cast<uint4*>(sW);
+    for (int i = tid; i < NLAY * KW * C * C / 8; i += INF_THREADS) dst[i] = src[i];
+    for (int i = tid; i < NLAY * C; i += INF_THREADS) sbias[i / C][i % C] = params[(i / C) * LAYER_SIZE + W_SIZE + i % C];
+  }
+  __syncthreads();
+  for (int col = blockIdx.x; col < ncols; col += gridDim.x) {
+    float* g = R + (long)col * NT_;
+    float am = 0.f, dm = 0.f;
+    for (int i = tid; i < NT_; i += INF_THREADS) {
+      const float v = g[i];
+      scol[i] = v;
+      am = fmaxf(am, fabsf(v));
+      if (i + 1 < NT_) dm = fmaxf(dm, fabsf(g[i + 1] - v));
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      am = fmaxf(am, __shfl_xor(am, o, 64));
+      dm = fmaxf(dm, __shfl_xor(dm, o, 64));
+    }
+    if (lane == 0) { sred[0][wave] = am; sred[1][wave] = dm; }
+    __syncthreads();
+    float A = 0.f, D = 0.f;
+#pragma unroll
+    for (int w = 0; w < INF_WAVES; ++w) { A = fmaxf(A, sred[0][w]); D = fmaxf(D, sred[1][w]); }
+    const bool fallback = A > 0.f && D / A * inv_w0dt > steep_max;  // block-uniform
+    if (tid == 0) flag[col] = fallback ? 1 : 0;
+    if (fallback || A == 0.f) { __syncthreads(); continue; }
+    const float gain = c * A * A;
+    for (int t0 = 0; t0 < NT_; t0 += TC) {
+      // features: row p is t = t0 - 16 + p
+      for (int p = tid; p < ROWS; p += INF_THREADS) {
+        const int t = t0 - 16 + p;
+        float x = 0.f;
+        if (t >= 0 && t < NT_) x = scol[t] / A;
+        uint4* row = reinterpret_cast<uint4*>(buf[0] + p * RS);
+        row[0] = make_uint4((uint32_t)f2bf(x) | ((uint32_t)f2bf(x * x) << 16), 0u, 0u, 0u);
+        row[1] = row[2] = row[3] = make_uint4(0u, 0u, 0u, 0u);
+      }
+      __syncthreads();
+      // hidden layers j = 1..3: outputs at rows 4j .. 4j + 16 n - 1
+#pragma unroll 1
+      for (int j = 1; j <= 3; ++j) {
+        const u16* in = buf[(j - 1) & 1];
+        u16* out = buf[j & 1];
+        const int first = 4 * j, ntile = (TC + 32 - 8 * j + 15) / 16;
+        bf16x8 a[2][KW];
+        load_a<2>(sW + (j - 1) * KW * C * C, lane, a);
+        for (int tt = wave; tt < ntile; tt += INF_WAVES) {
+          const int p0 = first + 16 * tt;
+          f32x4 acc[2];
+          conv_tile<2>(in, p0, lane, a, acc);
+          const int p = p0 + (lane & 15), t = t0 - 16 + p;
+          const bool inside = t >= 0 && t < NT_;
+#pragma unroll
+          for (int mt = 0; mt < 2; ++mt) {
+            const int co = 16 * mt + 4 * (lane >> 4);
+            float v[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) v[i] = inside ? fmaxf(acc[mt][i] + sbias[j - 1][co + i], 0.f) : 0.f;
+            *reinterpret_cast<uint2*>(out + p * RS + co) = pack4(v[0], v[1], v[2], v[3]);
+          }
+        }
+        __syncthreads();
+      }
+      // output layer: rows 16 .. 16 + TC - 1 are t0 .. t0 + TC - 1
+      {
+        bf16x8 a[2][KW];
+        load_a<1>(sW + 3 * KW * C * C, lane, a);
+        const u16* in = buf[1];
+        for (int tt = wave; tt < TC / 16; tt += INF_WAVES) {
+          const int p0 = 16 + 16 * tt;
+          f32x4 acc[2];
+          conv_tile<1>(in, p0, lane, a, acc);
+          if ((lane >> 4) == 0) {
+            const int t = t0 + 16 * tt + lane;
+            g[t] = scol[t] + gain * (acc[0][0] + sbias[3][0]);
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Training path: per-layer kernels over a batch of B columns, activations in
+// global memory as [B][nT][32] bf16.
+// ---------------------------------------------------------------------------
+constexpr int TT = 128;  // outputs per block of the per-layer kernels
+
+// Features and per-column amplitude: one block (256 threads) per column.
+__global__ __launch_bounds__(256) void k_net_feat(const float* __restrict__ u, int nT, u16* __restrict__ X0,
+                                                  float* __restrict__ amp) {
+  __shared__ float sred[4];
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const float* g = u + (long)b * nT;
+  float am = 0.f;
+  for (int t = tid; t < nT; t += 256) am = fmaxf(am, fabsf(g[t]));
+  for (int o = 32; o > 0; o >>= 1) am = fmaxf(am, __shfl_xor(am, o, 64));
+  if ((tid & 63) == 0) sred[tid >> 6] = am;
+  __syncthreads();
+  const float A = fmaxf(fmaxf(sred[0], sred[1]), fmaxf(sred[2], sred[3]));
+  if (tid == 0) amp[b] = A;
+  const float As = A > 0.f ? A : 1.f;
+  for (int t = tid; t < nT; t += 256) {
+    const float x = g[t] / As;
+    uint4* row = reinterpret_cast<uint4*>(X0 + ((long)b * nT + t) * C);
+    row[0] = make_uint4((uint32_t)f2bf(x) | ((uint32_t)f2bf(x * x) << 16), 0u, 0u, 0u);
+    row[1] = row[2] = row[3] = make_uint4(0u, 0u, 0u, 0u);
+  }
+}
+
+// MODE 0: out = bf16(relu(conv + bias)); MODE 1: y = conv[ch 0] + bias[0] (float);
+// MODE 2: out = bf16(conv * (mask > 0)) (input gradient through a relu).
+// grid (nT / TT, B), 256 threads.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_net_conv(const u16* __restrict__ in, int nT, const u16* __restrict__ W,
+                                                  const float* __restrict__ bias, const u16* __restrict__ mask,
+                                                  u16* __restrict__ out, float* __restrict__ y) {
+  __shared__ __align__(16) u16 win[(TT + 2 * HW) * RS];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int b = blockIdx.y, t0 = blockIdx.x * TT;
+  const long cb = (long)b * nT;
+  // window rows p = 0 .. TT+7 hold t = t0 - 4 + p (zero outside the column)
+  for (int i = tid; i < (TT + 2 * HW) * 4; i += 256) {
+    const int p = i >> 2, q = i & 3, t = t0 - HW + p;
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+    if (t >= 0 && t < nT) v = reinterpret_cast<const uint4*>(in + (cb + t) * C)[q];
+    reinterpret_cast<uint4*>(win + p * RS)[q] = v;
+  }
+  __syncthreads();
+  constexpr int NMT = MODE == 1 ? 1 : 2;
+  bf16x8 a[2][KW];
+  load_a<NMT>(W, lane, a);
+  for (int tt = wave; tt < TT / 16; tt += 4) {
+    f32x4 acc[2];
+    conv_tile<NMT>(win, HW + 16 * tt, lane, a, acc);
+    const int t = t0 + 16 * tt + (lane & 15);
+    if (t >= nT) continue;
+    if constexpr (MODE == 1) {
+      if ((lane >> 4) == 0) y[cb + t] = acc[0][0] + bias[0];
+    } else {
+#pragma unroll
+      for (int mt = 0; mt < 2; ++mt) {
+        const int co = 16 * mt + 4 * (lane >> 4);
+        float v[4];
+        if constexpr (MODE == 0) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) v[i] = fmaxf(acc[mt][i] + bias[co + i], 0.f);
+        } else {
+          const uint2 mk = *reinterpret_cast<const uint2*>(mask + (cb + t) * C + co);
+          const float m0 = bf2f((u16)(mk.x & 0xffff)), m1 = bf2f((u16)(mk.x >> 16));
+          const float m2 = bf2f((u16)(mk.y & 0xffff)), m3 = bf2f((u16)(mk.y >> 16));
+          v[0] = m0 > 0.f ? acc[mt][0] : 0.f;
+          v[1] = m1 > 0.f ? acc[mt][1] : 0.f;
+          v[2] = m2 > 0.f ? acc[mt][2] : 0.f;
+          v[3] = m3 > 0.f ? acc[mt][3] : 0.f;
+        }
+        *reinterpret_cast<uint2*>(out + (cb + t) * C + co) = pack4(v[0], v[1], v[2], v[3]);
+      }
+    }
+  }
+}
+
+// Loss (mean square over all samples) and its gradient, in channel 0 of dY:
+// y* = (u_next - u) / (c A^2).  grid (nT / 256, B), 256 threads.
+__global__ __launch_bounds__(256) void k_net_loss(const float* __restrict__ y, const float* __restrict__ u,
+                                                  const float* __restrict__ un, const float* __restrict__ amp,
+                                                  int nT, float c, float inv_count, u16* __restrict__ dY,
+                                                  double* __restrict__ loss) {
+  __shared__ double sred[4];
+  const int b = blockIdx.y, t = blockIdx.x * 256 + threadIdx.x;
+  double l2 = 0.0;
+  if (t < nT) {
+    const long i = (long)b * nT + t;
+    const float A = amp[b];
+    const float s = A > 0.f ? c * A * A : 1.f;
+    const float ys = A > 0.f ? (un[i] - u[i]) / s : 0.f;
+    const float r = y[i] - ys;
+    l2 = (double)r * (double)r;
+    uint4* row = reinterpret_cast<uint4*>(dY + i * C);
+    row[0] = make_uint4((uint32_t)f2bf(2.f * r * inv_count), 0u, 0u, 0u);
+    row[1] = row[2] = row[3] = make_uint4(0u, 0u, 0u, 0u);
+  }
+  for (int o = 32; o > 0; o >>= 1) l2 += __shfl_xor(l2, o, 64);
+  if ((threadIdx.x & 63) == 0) sred[threadIdx.x >> 6] = l2;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(loss, (sred[0] + sred[1] + sred[2] + sred[3]) * (double)inv_count);
+}
+
+// Weight and bias gradients of one layer: gW[co][ci][k] += sum_t D[t][co] X[t+k-4][ci].
+// grid (nT / TT, B), 256 threads: thread -> co = tid / 8, ci = 4 (tid % 8) .. +3.
+__global__ __launch_bounds__(256) void k_net_wgrad(const u16* __restrict__ Dg, const u16* __restrict__ X, int nT,
+                                                   float* __restrict__ gW, float* __restrict__ gb) {
+  __shared__ float sD[TT][C];
+  __shared__ float sX[TT + 2 * HW][C + 1];
+  const int tid = threadIdx.x, b = blockIdx.y, t0 = blockIdx.x * TT;
+  const long cb = (long)b * nT;
+  for (int i = tid; i < TT * C; i += 256) {
+    const int p = i / C, ch = i % C, t = t0 + p;
+    sD[p][ch] = t < nT ? bf2f(Dg[(cb + t) * C + ch]) : 0.f;
+  }
+  for (int i = tid; i < (TT + 2 * HW) * C; i += 256) {
+    const int p = i / C, ch = i % C, t = t0 - HW + p;
+    sX[p][ch] = (t >= 0 && t < nT) ? bf2f(X[(cb + t) * C + ch]) : 0.f;
+  }
+  __syncthreads();
+  const int co = tid >> 3, c0 = 4 * (tid & 7);
+  float acc[4][KW];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int k = 0; k < KW; ++k) acc[q][k] = 0.f;
+  float accb = 0.f;
+  for (int p = 0; p < TT; ++p) {
+    const float d = sD[p][co];
+    accb += d;
+#pragma unroll
+    for (int k = 0; k < KW; ++k)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) acc[q][k] = fmaf(d, sX[p + k][c0 + q], acc[q][k]);
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int k = 0; k < KW; ++k) atomicAdd(&gW[(co * C + c0 + q) * KW + k], acc[q][k]);
+  if ((tid & 7) == 0) atomicAdd(&gb[co], accb);
+}
+
+// Adam on the trainable entries (mask 1); masked entries stay at zero.
+__global__ void k_net_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                           float* __restrict__ v, const float* __restrict__ mask, int n, float lr, float b1, float b2,
+                           float eps, float bc1, float bc2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (mask[i] == 0.f) { p[i] = 0.f; return; }
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+}
+
+// bf16 weights for the matrix cores: Wm[j][k][co][ci] = W_j[co][ci][k] and the
+// flipped transpose Wt[j][k][ci][co] = W_j[co][ci][KW-1-k] (input gradients).
+__global__ void k_net_pack(const float* __restrict__ p, u16* __restrict__ Wm, u16* __restrict__ Wt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= NLAY * W_SIZE) return;
+  const int j = i / W_SIZE, r = i % W_SIZE;
+  const int co = r / (C * KW), ci = (r / KW) % C, k = r % KW;
+  const u16 w = f2bf(p[j * LAYER_SIZE + r]);
+  Wm[j * W_SIZE + (k * C + co) * C + ci] = w;
+  Wt[j * W_SIZE + ((KW - 1 - k) * C + ci) * C + co] = w;
+}
+
+}  // namespace net
+}  // namespace ash

@@ -67,6 +67,12 @@ class MarchPlan:
             pass
 
     # ---- march ---------------------------------------------------------------
+    def set_learned(self, net, steep_max: float = 2.5) -> None:
+        """Network update (a learned.LearnedStep, or None for KT) on columns with steepness <= steep_max."""
+        self.net = net
+        check(self.lib.ash_plan_set_learned(self.h, net.h if net is not None else None, float(steep_max)),
+              "ash_plan_set_learned")
+
     def set_dz(self, dz: float) -> None:
         check(self.lib.ash_plan_set_dz(self.h, float(dz)), "ash_plan_set_dz")
         self.dz = float(dz)

@@ -35,8 +35,8 @@ from .sources import SourceSlice
 class SolverParams:
     """All solver parameters, same names and defaults as ASS:35-76.
 
-    The two ``ash_*`` fields are extensions of this build (the reference has
-    no counterpart); their defaults reproduce the reference march exactly.
+    The ``ash_*`` fields are extensions of this build (the reference has no
+    counterpart); their defaults reproduce the reference march exactly.
     """
     dX: float = 0.0
     dY: float = 0.0
@@ -70,6 +70,8 @@ class SolverParams:
     # --- extensions ---
     ash_subcycle: str = 'off'       # 'off' | 'column': per-column CFL sub-cycling of KT
     ash_cfl_target: float = 0.5
+    ash_learned: object = None      # a learned.LearnedStep: network update on smooth columns
+    ash_steep_max: float = 2.5      # KT on columns steeper than this (1 = sinusoid at f0)
 
 
 class StabilityError(RuntimeError):
@@ -124,6 +126,8 @@ def angular_spectrum_solve(
     dZ = max(dZ, params.dZmin)
 
     plan = MarchPlan((nX, nY, nT), params, params.ash_subcycle, params.ash_cfl_target)
+    if params.ash_learned is not None:
+        plan.set_learned(params.ash_learned, params.ash_steep_max)
     if _plan_out is not None:
         _plan_out.append(plan)
     screens = _upload_screens(params.phaseScreens)

@@ -117,8 +117,8 @@ def main():
 
     # per-kernel launch times with HIP events on the launch stream
     import ctypes
-    msv = (ctypes.c_float * 8)()
-    cnt = (ctypes.c_int32 * 8)()
+    msv = (ctypes.c_float * len(_lib.KERNEL_KINDS))()
+    cnt = (ctypes.c_int32 * len(_lib.KERNEL_KINDS))()
     io.row = row
     _lib.check(plan.lib.ash_profile_step(plan.h, ctypes.byref(io), P, msv, cnt, _lib.current_stream()),
                "ash_profile_step")

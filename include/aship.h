@@ -184,7 +184,8 @@ int ash_apply_phase_screen(ash_plan* plan, const float* in_dev, float* out_dev, 
 #define ASH_K_COL_TAPER 5   /* t column: irfft, taper, rfft */
 #define ASH_K_COL_FINAL 6   /* t column: irfft, sources, reductions */
 #define ASH_K_COL_OTHER 7
-#define ASH_K_COUNT 8
+#define ASH_K_NET 8         /* learned nonlinear step (network inference) */
+#define ASH_K_COUNT 9
 
 /* Timing aid with no counterpart in the reference (whose only timing is the
  * wall clock of ASS:1411, :1552): runs n_iter steps like ash_step, without the
@@ -197,6 +198,57 @@ int ash_profile_step(ash_plan* plan, const ash_step_io* io, int32_t n_iter, floa
 
 /* Bytes of device workspace held by the plan. */
 int64_t ash_plan_workspace_bytes(const ash_plan* plan);
+
+/* ---------------------------------------------------------------------------
+ * Learned nonlinear step.  No counterpart in the reference: a 4-layer 1-D conv
+ * net over t (kernel 9, 32 channels, bf16 activations and weights on the
+ * matrix cores, float32 accumulation) that stands in for the KT update over
+ * one dZ (_kt_flux, ASS:833-839) on smooth columns.  With A = max|u| of a
+ * column, the net maps x = u/A (features x, x^2) to y and the step is
+ * u += c A^2 y with c = dZ N / dT.  Columns whose steepness
+ * max|u[t+1]-u[t]| / (A 2 pi f0 dT) exceeds a threshold take the KT update.
+ * Parameters: float32, ash_net_param_count() entries, four layers of
+ * W[co][ci][k] (32 x 32 x 9) followed by b[co] (32); layer 1 reads input
+ * channels 0-1 only and layer 4 writes output channel 0 only (the other
+ * entries are held at zero).
+ * ------------------------------------------------------------------------- */
+typedef struct ash_net ash_net;
+
+int64_t ash_net_param_count(void);
+
+/* nT: column length (256, 512, 1024 or 2048); max_cols: largest training
+ * batch (columns) for ash_net_forward / ash_net_loss_grad. */
+int ash_net_create(int32_t nT, int32_t max_cols, ash_net** out);
+void ash_net_destroy(ash_net* net);
+
+/* Copy float32 parameters (device) in or out. */
+int ash_net_set_params(ash_net* net, const float* params_dev, void* stream);
+int ash_net_get_params(ash_net* net, float* params_dev, void* stream);
+
+/* Training-path forward on ncols columns u[ncols][nT]: y[ncols][nT] is the
+ * normalised output, amp[ncols] the column amplitudes A. */
+int ash_net_forward(ash_net* net, const float* u_dev, int32_t ncols, float* y_dev, float* amp_dev, void* stream);
+
+/* Mean-square loss of y against y* = (u_next - u) / (c A^2) over all samples,
+ * added to *loss_dev (double), and its gradient in the parameters written to
+ * grads_dev (float32, ash_net_param_count() entries). */
+int ash_net_loss_grad(ash_net* net, const float* u_dev, const float* u_next_dev, int32_t ncols, double c,
+                      double* loss_dev, float* grads_dev, void* stream);
+
+/* One Adam update (step counts from 1) of the parameters with grads_dev. */
+int ash_net_adam(ash_net* net, const float* grads_dev, float lr, float beta1, float beta2, float eps,
+                 int32_t step, void* stream);
+
+/* Fused inference in place on real columns u[ncols][nT]: u += c A^2 y for the
+ * columns whose steepness is at most steep_max; flag[c] = 1 marks the others
+ * (left unchanged). */
+int ash_net_apply(ash_net* net, float* u_dev, int32_t ncols, double c, double steep_max, double f0_dT,
+                  int32_t* flag_dev, void* stream);
+
+/* Use the network for the nonlinear update of the plan's steps (null: the KT
+ * update).  Split-step KT plans only; nT must match.  The plan keeps a real
+ * scratch field (nX nY nT float32) while a network is set. */
+int ash_plan_set_learned(ash_plan* plan, ash_net* net, double steep_max);
 
 #ifdef __cplusplus
 }
