@@ -112,7 +112,7 @@ struct ash_plan {
   size_t ev_next = 0;
   // learned nonlinear step (ash_plan_set_learned)
   ash_net* net = nullptr;
-  double steep_max = 0.0;
+  double dpdt_max = 0.0;     // KT where max|dp/dt| of a column exceeds this (Pa/s)
   float* rscratch = nullptr;  // real field between the column passes [nX*nY*nT]
   int32_t* nflag = nullptr;   // 1 where a column took the KT update [nX*nY]
 };
@@ -504,15 +504,15 @@ EpiDesc epi_base(const ash_plan* p) {
   return e;
 }
 
-int net_apply(ash_net* n, float* u, int ncols, double c, double steep_max, double f0_dT, int32_t* flag,
+int net_apply(ash_net* n, float* u, int ncols, double c, double dpdt_max, double dT, int32_t* flag,
               hipStream_t s) {
   const int grid = std::min(ncols, std::max(n->n_cu, 1));  // one 127-131 KB LDS block per CU
-  const float inv = (float)(1.0 / (2.0 * M_PI * f0_dT));
+  const float inv = (float)(1.0 / dT);
   switch (n->nT) {
-    case 256: hipLaunchKernelGGL(net::k_net_infer<256>, dim3(grid), dim3(net::INF_THREADS), 0, s, u, ncols, n->Wm, n->params, (float)c, (float)steep_max, inv, flag); break;
-    case 512: hipLaunchKernelGGL(net::k_net_infer<512>, dim3(grid), dim3(net::INF_THREADS), 0, s, u, ncols, n->Wm, n->params, (float)c, (float)steep_max, inv, flag); break;
-    case 1024: hipLaunchKernelGGL(net::k_net_infer<1024>, dim3(grid), dim3(net::INF_THREADS), 0, s, u, ncols, n->Wm, n->params, (float)c, (float)steep_max, inv, flag); break;
-    case 2048: hipLaunchKernelGGL(net::k_net_infer<2048>, dim3(grid), dim3(net::INF_THREADS), 0, s, u, ncols, n->Wm, n->params, (float)c, (float)steep_max, inv, flag); break;
+    case 256: hipLaunchKernelGGL(net::k_net_infer<256>, dim3(grid), dim3(net::INF_THREADS), 0, s, u, ncols, n->Wm, n->params, (float)c, (float)dpdt_max, inv, flag); break;
+    case 512: hipLaunchKernelGGL(net::k_net_infer<512>, dim3(grid), dim3(net::INF_THREADS), 0, s, u, ncols, n->Wm, n->params, (float)c, (float)dpdt_max, inv, flag); break;
+    case 1024: hipLaunchKernelGGL(net::k_net_infer<1024>, dim3(grid), dim3(net::INF_THREADS), 0, s, u, ncols, n->Wm, n->params, (float)c, (float)dpdt_max, inv, flag); break;
+    case 2048: hipLaunchKernelGGL(net::k_net_infer<2048>, dim3(grid), dim3(net::INF_THREADS), 0, s, u, ncols, n->Wm, n->params, (float)c, (float)dpdt_max, inv, flag); break;
     default: return fail(ASH_E_UNSUPPORTED, "network column length must be 256, 512, 1024 or 2048");
   }
   return ASH_OK;
@@ -527,8 +527,7 @@ void learned_nl(ash_plan* p, ColDesc cd, double dz, hipStream_t s) {
   column(p, cd, s);
   {
     ProfScope ps(p, ASH_K_NET, s);
-    net_apply(p->net, p->rscratch, p->g.nX * p->g.nY, dz * p->N / p->d.dT, p->steep_max, p->d.f0 * p->d.dT,
-              p->nflag, s);
+    net_apply(p->net, p->rscratch, p->g.nX * p->g.nY, dz * p->N / p->d.dT, p->dpdt_max, p->d.dT, p->nflag, s);
   }
   cd.mode = COL_RAWNL;
   cd.taper = 0;
@@ -1076,17 +1075,17 @@ int ash_net_adam(ash_net* n, const float* grads, float lr, float b1, float b2, f
   return check_launch();
 }
 
-int ash_net_apply(ash_net* n, float* u, int32_t ncols, double c, double steep_max, double f0_dT, int32_t* flag,
+int ash_net_apply(ash_net* n, float* u, int32_t ncols, double c, double dpdt_max, double dT, int32_t* flag,
                   void* stream) {
   if (!n || !u || !flag) return fail(ASH_E_ARG, "null argument");
   if (ncols < 1) return fail(ASH_E_ARG, "ncols must be positive");
-  if (!(f0_dT > 0)) return fail(ASH_E_ARG, "f0 dT must be positive");
-  int rc = net_apply(n, u, ncols, c, steep_max, f0_dT, flag, (hipStream_t)stream);
+  if (!(dT > 0)) return fail(ASH_E_ARG, "dT must be positive");
+  int rc = net_apply(n, u, ncols, c, dpdt_max, dT, flag, (hipStream_t)stream);
   if (rc) return rc;
   return check_launch();
 }
 
-int ash_plan_set_learned(ash_plan* p, ash_net* n, double steep_max) {
+int ash_plan_set_learned(ash_plan* p, ash_net* n, double dpdt_max) {
   if (!p) return fail(ASH_E_ARG, "null plan");
   if (n) {
     if (!p->d.useSplitStep || p->d.fluxScheme != 1)
@@ -1103,7 +1102,7 @@ int ash_plan_set_learned(ash_plan* p, ash_net* n, double steep_max) {
     }
   }
   p->net = n;
-  p->steep_max = steep_max;
+  p->dpdt_max = dpdt_max;
   // the captured step graph holds the other update
   if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
   if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }

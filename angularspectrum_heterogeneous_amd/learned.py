@@ -3,8 +3,8 @@
 The reference has no learned component.  A 4-layer 1-D conv net over t
 (kernel 9, 32 channels, bf16 weights and activations on the matrix cores,
 float32 accumulation) stands in for the KT update over one dZ (ASS:833-839)
-on smooth columns; plain KT stays the default, the fallback for steep columns
-and the judge of the net's error.  Training pairs (column before, column after
+on smooth columns; plain KT stays the default, the fallback wherever a
+column's max|dp/dt| exceeds a threshold, and the judge of the net's error.  Training pairs (column before, column after
 one KT step) come from the plain gfx950 KT kernel itself, and the network is
 trained on the GPU by the library's own forward, backward and Adam kernels.
 
@@ -121,13 +121,13 @@ class LearnedStep:
                                     current_stream()), "ash_net_adam")
 
     # -- inference ---------------------------------------------------------
-    def apply(self, u: torch.Tensor, c: float, steep_max: float, f0_dT: float) -> torch.Tensor:
-        """In place u += c A^2 y on columns with steepness <= steep_max; returns the fallback flags."""
+    def apply(self, u: torch.Tensor, c: float, dpdt_max: float, dT: float) -> torch.Tensor:
+        """In place u += c A^2 y on columns with max|du/dt| <= dpdt_max (Pa/s); returns the fallback flags."""
         if u.device.type != "cuda" or u.dtype != torch.float32 or u.shape[-1] != self.nT or not u.is_contiguous():
             raise ValueError(f"expected a contiguous CUDA float32 tensor with last dimension {self.nT}")
         ncols = u.numel() // self.nT
         flags = torch.empty(ncols, dtype=torch.int32, device=self.dev)
-        check(self.lib.ash_net_apply(self.h, u.data_ptr(), ncols, float(c), float(steep_max), float(f0_dT),
+        check(self.lib.ash_net_apply(self.h, u.data_ptr(), ncols, float(c), float(dpdt_max), float(dT),
                                      flags.data_ptr(), current_stream()), "ash_net_apply")
         return flags
 

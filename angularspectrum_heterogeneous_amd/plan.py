@@ -67,10 +67,10 @@ class MarchPlan:
             pass
 
     # ---- march ---------------------------------------------------------------
-    def set_learned(self, net, steep_max: float = 2.5) -> None:
-        """Network update (a learned.LearnedStep, or None for KT) on columns with steepness <= steep_max."""
+    def set_learned(self, net, dpdt_max: float) -> None:
+        """Network update (a learned.LearnedStep, or None for KT) on columns with max|dp/dt| <= dpdt_max (Pa/s)."""
         self.net = net
-        check(self.lib.ash_plan_set_learned(self.h, net.h if net is not None else None, float(steep_max)),
+        check(self.lib.ash_plan_set_learned(self.h, net.h if net is not None else None, float(dpdt_max)),
               "ash_plan_set_learned")
 
     def set_dz(self, dz: float) -> None:

@@ -71,7 +71,7 @@ class SolverParams:
     ash_subcycle: str = 'off'       # 'off' | 'column': per-column CFL sub-cycling of KT
     ash_cfl_target: float = 0.5
     ash_learned: object = None      # a learned.LearnedStep: network update on smooth columns
-    ash_steep_max: float = 2.5      # KT on columns steeper than this (1 = sinusoid at f0)
+    ash_dpdt_max: float = 6e12      # KT on columns whose max|dp/dt| exceeds this (Pa/s)
 
 
 class StabilityError(RuntimeError):
@@ -127,7 +127,7 @@ def angular_spectrum_solve(
 
     plan = MarchPlan((nX, nY, nT), params, params.ash_subcycle, params.ash_cfl_target)
     if params.ash_learned is not None:
-        plan.set_learned(params.ash_learned, params.ash_steep_max)
+        plan.set_learned(params.ash_learned, params.ash_dpdt_max)
     if _plan_out is not None:
         _plan_out.append(plan)
     screens = _upload_screens(params.phaseScreens)

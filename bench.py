@@ -13,7 +13,15 @@ Also reported:
   step_roofline - the step's algorithmic bytes (104 B per voxel-z-step) at the
                   measured step rate, as a fraction of 8 TB/s;
   cpu_baseline  - the FP32 NumPy/SciPy restatement of the reference march timed
-                  on this host's cores over a bounded sample of the same workload.
+                  on this host's cores over a bounded sample of the same workload;
+  learned       - the same march with the learned nonlinear step (bf16 conv net
+                  on the matrix cores, KT where a column's max|dp/dt| exceeds a
+                  threshold): for a sweep of thresholds, rel-L2 of the full run
+                  against plain KT and the share of columns left to KT; at the
+                  largest threshold within 1e-3 on the field, its step time, the
+                  ratio to the plain step, and rel-L2 of the full run against
+                  plain KT for field, pnp, ppp, pI and pIloss.  The network is
+                  trained first, on the GPU, on 200k KT (before, after) pairs.
 """
 from __future__ import annotations
 
@@ -56,36 +64,18 @@ def cpu_baseline(steps: int):
             "step_s": [round(s, 3) for s in tim["step_s"]]}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--profile-iters", type=int, default=5)
-    ap.add_argument("--cpu-steps", type=int, default=2)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--copy-calib", action="store_true",
-                    help="also run 3 elementwise passes over 512 MiB (known HBM bytes) to scale PMC byte counters")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(--gpus must be 1)", file=sys.stderr)
-        sys.exit(2)
-
-    import numpy as np
+def timed_march(field_np, params, dz, K, Wm, P, net=None, dpdt_max=0.0):
+    """W warm-up steps, K timed steps (synchronised on both sides), then P steps with
+    HIP events around every launch.  Returns (seconds, per-kernel dict, max |p| seen)."""
+    import ctypes
     import torch
-    from angularspectrum_heterogeneous_amd import _lib, workloads as W
+    from angularspectrum_heterogeneous_amd import _lib
     from angularspectrum_heterogeneous_amd.plan import MarchPlan
-    from angularspectrum_heterogeneous_amd.solver import SolverParams
-
-    field_np, kw = W.config2()
-    params = SolverParams(**kw)
     nX, nY, nT = field_np.shape
-    vox = nX * nY * nT
-    dz = params.dZmin
     plan = MarchPlan(field_np.shape, params)
     plan.set_dz(dz)
-    K, Wm, P = args.steps, args.warmup, args.profile_iters
+    if net is not None:
+        plan.set_learned(net, dpdt_max)
     rows = K + Wm + P + 1
     dev = torch.device("cuda")
     field = torch.from_numpy(field_np).to(dev)
@@ -110,13 +100,8 @@ def main():
         plan.step(io)
         row += 1
     torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    value = vox * K / elapsed
-    ms_step = elapsed / K * 1e3
-
+    elapsed = time.perf_counter() - t0
     # per-kernel launch times with HIP events on the launch stream
-    import ctypes
     msv = (ctypes.c_float * len(_lib.KERNEL_KINDS))()
     cnt = (ctypes.c_int32 * len(_lib.KERNEL_KINDS))()
     io.row = row
@@ -127,6 +112,103 @@ def main():
     for k, name in enumerate(_lib.KERNEL_KINDS):
         if cnt[k]:
             kernels[name] = {"avg_ms": round(float(msv[k]), 5), "launches_per_step": int(cnt[k])}
+    fin = float(stats[: row + 1].view(torch.float32).max().item())
+    plan.close()
+    return elapsed, kernels, fin
+
+
+def learned_leg(field_np, kw, dz, K, Wm, P, plain_ms, n_pairs, epochs):
+    """The learned nonlinear step on the same workload: KT training pairs and training on
+    the GPU, the timed march with the network, and rel-L2 of the full run against plain KT."""
+    import numpy as np
+    import torch
+    from angularspectrum_heterogeneous_amd import learned as L
+    from angularspectrum_heterogeneous_amd.solver import SolverParams, angular_spectrum_solve
+    nT = field_np.shape[2]
+    sp = SolverParams(**kw)
+    c = dz * L.burgers_coefficient(sp) / sp.dT
+    t0 = time.perf_counter()
+    before, after = L.generate_pairs(sp, nT, dz, n_pairs=n_pairs, stride=8, pairs_per_column=50, seed=0)
+    torch.cuda.synchronize()
+    t_gen = time.perf_counter() - t0
+    net = L.LearnedStep(nT, max_cols=256, seed=0)
+    t0 = time.perf_counter()
+    hist = L.train(net, before, after, c, epochs=epochs, batch=256, lr=2e-3)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    del before, after
+    torch.cuda.empty_cache()
+    ref = angular_spectrum_solve(field_np, SolverParams(**kw), verbose=False)
+    names = ("field", "pnp", "ppp", "pI", "pIloss")
+
+    def rel(a, b):
+        a = np.asarray(a, np.float64)
+        b = np.asarray(b, np.float64)
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+    sweep = []
+    for thr in (1e30, 1.2e13, 6e12, 3e12, 1.5e12):
+        shares = []
+
+        def cb(cc, z, f, thr=thr):
+            u = f.reshape(-1, nT)
+            A = u.abs().amax(1)
+            live = A > 1e-3 * A.max()
+            D = (u[:, 1:] - u[:, :-1]).abs().amax(1) / sp.dT
+            shares.append(float((D[live] > thr).float().mean()))
+
+        got = angular_spectrum_solve(field_np, SolverParams(**kw, ash_learned=net, ash_dpdt_max=thr),
+                                     verbose=False, per_step_callback=cb)
+        sweep.append({"dpdt_max": thr, "rel_l2_vs_kt": {n: rel(got[i], ref[i]) for i, n in enumerate(names)},
+                      "kt_share_of_beam_columns": float(np.mean(shares))})
+    ok = [e for e in sweep if e["rel_l2_vs_kt"]["field"] <= 1e-3]
+    pick = ok[0] if ok else sweep[-1]
+    elapsed, kernels, _ = timed_march(field_np, SolverParams(**kw), dz, K, Wm, P, net=net,
+                                      dpdt_max=pick["dpdt_max"])
+    ms = elapsed / K * 1e3
+    return {"ms_per_step": ms, "value": field_np.size * K / elapsed, "speedup_vs_plain": plain_ms / ms,
+            "dpdt_max": pick["dpdt_max"], "rel_l2_vs_kt": pick["rel_l2_vs_kt"],
+            "meets_1e-3": bool(ok), "steps_compared": int(len(ref[5])), "sweep": sweep,
+            "kernels": kernels, "train": {"pairs": n_pairs, "epochs": epochs, "batch": 256,
+                                          "loss_first_epoch": hist[0], "loss_last_epoch": hist[-1],
+                                          "pairs_s": round(t_gen, 2), "train_s": round(t_train, 2)},
+            "net": "4 x conv1d(k=9, 32 ch) over t, bf16 weights/activations on v_mfma_f32_16x16x32_bf16, "
+                   "fp32 accumulate; trained on the GPU by the library's own kernels"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--profile-iters", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned nonlinear step leg")
+    ap.add_argument("--learned-pairs", type=int, default=200_000)
+    ap.add_argument("--learned-epochs", type=int, default=3)
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run 3 elementwise passes over 512 MiB (known HBM bytes) to scale PMC byte counters")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(--gpus must be 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from angularspectrum_heterogeneous_amd import _lib, workloads as W
+    from angularspectrum_heterogeneous_amd.solver import SolverParams
+
+    field_np, kw = W.config2()
+    params = SolverParams(**kw)
+    nX, nY, nT = field_np.shape
+    vox = nX * nY * nT
+    dz = params.dZmin
+    K, Wm, P = args.steps, args.warmup, args.profile_iters
+    elapsed, kernels, fin = timed_march(field_np, params, dz, K, Wm, P)
+    value = vox * K / elapsed
+    ms_step = elapsed / K * 1e3
     dom = max(kernels, key=lambda n: kernels[n]["avg_ms"] * kernels[n]["launches_per_step"])
     dom_s = kernels[dom]["avg_ms"] * 1e-3
     ab = algorithmic_bytes_per_launch(dom, vox, nX * nY, nT)
@@ -145,7 +227,7 @@ def main():
     step_bytes = 104.0 * vox  # 13 sweeps x 8 B at this configuration
     step_frac = step_bytes / (elapsed / K) / 1e9 / HBM_PEAK_GBS
 
-    fin = float(stats[: row + 1].view(torch.float32).max().item())
+    dev = torch.device("cuda")
     if args.copy_calib:
         # an elementwise kernel of known traffic: reads 512 MiB, writes 512 MiB
         a = torch.ones(128 * 1024 * 1024, dtype=torch.float32, device=dev)
@@ -178,9 +260,11 @@ def main():
         "step_roofline": {"alg_bytes_per_voxel_step": 104, "frac": round(step_frac, 4)},
         "kernels": kernels,
     }
+    if not args.no_learned:
+        result["learned"] = learned_leg(field_np, kw, dz, K, Wm, P, ms_step, args.learned_pairs,
+                                        args.learned_epochs)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(args.cpu_steps)
-    plan.close()
     print(json.dumps(result), flush=True)
 
 

@@ -205,8 +205,8 @@ int64_t ash_plan_workspace_bytes(const ash_plan* plan);
  * matrix cores, float32 accumulation) that stands in for the KT update over
  * one dZ (_kt_flux, ASS:833-839) on smooth columns.  With A = max|u| of a
  * column, the net maps x = u/A (features x, x^2) to y and the step is
- * u += c A^2 y with c = dZ N / dT.  Columns whose steepness
- * max|u[t+1]-u[t]| / (A 2 pi f0 dT) exceeds a threshold take the KT update.
+ * u += c A^2 y with c = dZ N / dT.  Columns whose largest slope
+ * max|u[t+1]-u[t]| / dT exceeds a threshold (Pa/s) take the KT update.
  * Parameters: float32, ash_net_param_count() entries, four layers of
  * W[co][ci][k] (32 x 32 x 9) followed by b[co] (32); layer 1 reads input
  * channels 0-1 only and layer 4 writes output channel 0 only (the other
@@ -240,15 +240,16 @@ int ash_net_adam(ash_net* net, const float* grads_dev, float lr, float beta1, fl
                  int32_t step, void* stream);
 
 /* Fused inference in place on real columns u[ncols][nT]: u += c A^2 y for the
- * columns whose steepness is at most steep_max; flag[c] = 1 marks the others
- * (left unchanged). */
-int ash_net_apply(ash_net* net, float* u_dev, int32_t ncols, double c, double steep_max, double f0_dT,
+ * columns whose max|u[t+1]-u[t]| / dT is at most dpdt_max; flag[c] = 1 marks
+ * the others (left unchanged).  Within a column, a 256-sample chunk whose
+ * window (16 samples of halo each side) stays below 1e-4 A is left unchanged. */
+int ash_net_apply(ash_net* net, float* u_dev, int32_t ncols, double c, double dpdt_max, double dT,
                   int32_t* flag_dev, void* stream);
 
 /* Use the network for the nonlinear update of the plan's steps (null: the KT
  * update).  Split-step KT plans only; nT must match.  The plan keeps a real
  * scratch field (nX nY nT float32) while a network is set. */
-int ash_plan_set_learned(ash_plan* plan, ash_net* net, double steep_max);
+int ash_plan_set_learned(ash_plan* plan, ash_net* net, double dpdt_max);
 
 #ifdef __cplusplus
 }

@@ -173,17 +173,27 @@ def loss_and_grads(params: np.ndarray, acts, y: np.ndarray, ystar: np.ndarray, m
     return loss, grad
 
 
-def apply(params: np.ndarray, u: np.ndarray, c: float, quant: bool = True) -> np.ndarray:
-    """u_new = u + c A^2 y for every column of u (B, T)."""
+def apply(params: np.ndarray, u: np.ndarray, c: float, quant: bool = True, chunk: int = 256, halo: int = 16,
+          quiet: float = 1e-4) -> np.ndarray:
+    """u_new = u + c A^2 y for every column of u (B, T).
+
+    As the fused kernel: an output chunk [t0, t0 + chunk) whose window
+    [t0 - halo, t0 + chunk + halo) has max |u / A| below ``quiet`` is unchanged."""
+    u = np.asarray(u, dtype=np.float32)
     X0, A = features(u, quant)
     _, y = forward(params, X0, quant)
-    return (u.astype(np.float64) + c * (A.astype(np.float64) ** 2)[:, None] * y).astype(np.float32)
+    As = np.where(A > 0, A, np.float32(1.0)).astype(np.float32)
+    x = np.abs(u / As[:, None])
+    keep = np.zeros(u.shape, dtype=bool)
+    T = u.shape[1]
+    for t0 in range(0, T, chunk):
+        lo, hi = max(t0 - halo, 0), min(t0 + chunk + halo, T)
+        keep[:, t0:t0 + chunk] = (np.max(x[:, lo:hi], axis=1) >= quiet)[:, None]
+    inc = c * (A.astype(np.float64) ** 2)[:, None] * y * keep
+    return (u.astype(np.float64) + inc).astype(np.float32)
 
 
-def steepness(u: np.ndarray, dT: float, f0: float) -> np.ndarray:
-    """Shock indicator max_t |u[t+1] - u[t]| / (A w0 dT), 1 for a sampled sinusoid of frequency f0."""
-    u = np.asarray(u, dtype=np.float64)
-    A = np.max(np.abs(u), axis=1)
-    g = np.max(np.abs(np.diff(u, axis=1)), axis=1)
-    w = 2.0 * np.pi * f0 * dT
-    return np.where(A > 0, g / np.where(A > 0, A, 1.0) / w, 0.0)
+def max_dpdt(u: np.ndarray, dT: float) -> np.ndarray:
+    """Fallback indicator max_t |u[t+1] - u[t]| / dT per column (Pa/s)."""
+    u = np.asarray(u, dtype=np.float32)
+    return np.max(np.abs(np.diff(u, axis=1)), axis=1).astype(np.float64) / dT

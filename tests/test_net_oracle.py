@@ -74,8 +74,20 @@ def test_trainable_mask_and_features():
     assert A[0] == 4.0
     assert np.allclose(X[0, :, 0], [0.0, 0.5, -1.0, 0.25]) and np.allclose(X[0, :, 1], [0.0, 0.25, 1.0, 0.0625])
     assert not X[..., 2:].any()
-    # steepness is 1 for a sampled sinusoid at f0 (up to the sampling of its maximum slope)
-    dT, f0 = 4e-8, 1e6
+    # the fallback indicator of a sampled sinusoid A sin(w t) is 2 A sin(w dT / 2) / dT (~ A w)
+    dT, f0, amp = 4e-8, 1e6, 3e5
     t = np.arange(4096) * dT
-    s = NO.steepness(np.sin(2 * np.pi * f0 * t)[None, :], dT, f0)
-    assert abs(s[0] - 1.0) < 1e-3
+    s = NO.max_dpdt((amp * np.sin(2 * np.pi * f0 * t))[None, :], dT)
+    assert abs(s[0] / (2 * amp * np.sin(np.pi * f0 * dT) / dT) - 1.0) < 1e-5
+
+
+def test_apply_leaves_quiet_chunks_unchanged():
+    rng = np.random.default_rng(4)
+    p = NO.init_params(1) + (rng.normal(0, 0.05, NO.PARAM_COUNT) * NO.trainable_mask()).astype(np.float32)
+    p[3 * NO.LAYER_SIZE + NO.W_SIZE] = 0.3  # output bias: a nonzero y everywhere
+    u = np.zeros((2, 512), np.float32)
+    u[:, 300:340] = rng.normal(size=(2, 40)).astype(np.float32)
+    out = NO.apply(p, u, 1e-3)
+    # chunk 0 (outputs [0, 256), window [0, 272)) sees only zeros: unchanged
+    assert np.array_equal(out[:, :256], u[:, :256])
+    assert not np.array_equal(out[:, 256:512], u[:, 256:512])
