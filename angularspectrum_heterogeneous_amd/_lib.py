@@ -76,6 +76,11 @@ _PROTOS = {
     "ash_profile_step": ([c_vp, ctypes.POINTER(AshStepIO), c_i32, ctypes.POINTER(ctypes.c_float),
                           ctypes.POINTER(c_i32), c_vp], c_i32),
     "ash_plan_workspace_bytes": ([c_vp], ctypes.c_int64),
+    "ash_plan_autotune": ([c_vp, c_vp, c_vp], c_i32),
+    "ash_plan_set_launch": ([c_vp, c_vp], c_i32),
+    "ash_batch_create": ([ctypes.POINTER(c_vp)], c_i32),
+    "ash_batch_destroy": ([c_vp], None),
+    "ash_batch_step": ([c_vp, c_vp, c_vp, c_i32, c_vp], c_i32),
     "ash_net_param_count": ([], ctypes.c_int64),
     "ash_net_create": ([c_i32, c_i32, ctypes.POINTER(c_vp)], c_i32),
     "ash_net_destroy": ([c_vp], None),

@@ -103,7 +103,10 @@ struct ash_plan {
   hipGraphExec_t gexec = nullptr;
   hipStream_t cap_stream = nullptr;
   ash_step_io gkey{};
-  double gkey_dz = 0.0;
+  int epoch = 0;  // bumped when the step sequence changes (captured graphs are stale)
+  // launch configuration (ash_plan_autotune)
+  int rvx = 0, rvy = 0;  // radix order of the x- and y-line transforms
+  int col_grid = 2048;   // blocks of the persistent column kernels
   bool use_graph = true;
   // per-kernel timing (ash_profile_step): events around each launch
   bool prof_on = false;
@@ -130,6 +133,15 @@ struct ash_net {
   float* y = nullptr;       // [max_cols][nT]
   float* amp = nullptr;     // [max_cols]
   std::vector<void*> allocs;
+};
+
+struct ash_batch {
+  std::vector<ash_plan*> plans;    // set of the captured graph
+  std::vector<ash_step_io> keys;
+  std::vector<int> epochs;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  hipStream_t cap = nullptr;
 };
 
 namespace {
@@ -379,19 +391,26 @@ struct ProfScope {
 };
 
 // ---- launch helpers --------------------------------------------------------
+// rvx / rvy: radix order of the x-line and y-line transforms (Radix<L, V>).
 template <int L>
 void launch_xfwd_L(ash_plan* p, hipStream_t s) {
-  hipLaunchKernelGGL(k_xfft_fwd<L>, dim3(p->g.nY * p->g.nfb), dim3(LNT), 0, s, p->spec, p->T, p->g);
+  const dim3 grid(p->g.nY * p->g.nfb);
+  if (p->rvx) hipLaunchKernelGGL((k_xfft_fwd<L, 1>), grid, dim3(LNT), 0, s, p->spec, p->T, p->g);
+  else hipLaunchKernelGGL((k_xfft_fwd<L, 0>), grid, dim3(LNT), 0, s, p->spec, p->T, p->g);
 }
 template <int L>
 void launch_ypass_L(ash_plan* p, const OpDesc& op, hipStream_t s) {
   const dim3 grid((p->g.nX / 2) * p->g.nfb);  // one block per (x row pair, f group)
-  if (op.precise) hipLaunchKernelGGL((k_ypass<L, true>), grid, dim3(LNT), 0, s, p->spec, p->T, p->g, op);
-  else hipLaunchKernelGGL((k_ypass<L, false>), grid, dim3(LNT), 0, s, p->spec, p->T, p->g, op);
+  // the FP64-operator reference runs with the default radix order
+  if (op.precise) hipLaunchKernelGGL((k_ypass<L, true, 0>), grid, dim3(LNT), 0, s, p->spec, p->T, p->g, op);
+  else if (p->rvy) hipLaunchKernelGGL((k_ypass<L, false, 1>), grid, dim3(LNT), 0, s, p->spec, p->T, p->g, op);
+  else hipLaunchKernelGGL((k_ypass<L, false, 0>), grid, dim3(LNT), 0, s, p->spec, p->T, p->g, op);
 }
 template <int L, int EPI>
 void launch_xinv_L(ash_plan* p, const EpiDesc& e, hipStream_t s) {
-  hipLaunchKernelGGL((k_xfft_inv<L, EPI>), dim3(p->g.nY * p->g.nfb), dim3(LNT), 0, s, p->spec, p->T, p->g, e);
+  const dim3 grid(p->g.nY * p->g.nfb);
+  if (p->rvx) hipLaunchKernelGGL((k_xfft_inv<L, EPI, 1>), grid, dim3(LNT), 0, s, p->spec, p->T, p->g, e);
+  else hipLaunchKernelGGL((k_xfft_inv<L, EPI, 0>), grid, dim3(LNT), 0, s, p->spec, p->T, p->g, e);
 }
 template <int L>
 void launch_ysa_L(ash_plan* p, const ScreenDesc& sd, hipStream_t s) {
@@ -402,7 +421,7 @@ void launch_col_MN(ash_plan* p, const ColDesc& cd, hipStream_t s) {
   const int ncols = p->g.nX * p->g.nY;
   const int per_block = ColCfg<M>::WPB * ColCfg<M>::CPW;
   int blocks = (ncols + per_block - 1) / per_block;
-  blocks = std::min(blocks, 2048);  // persistent: each wave walks several columns
+  blocks = std::min(blocks, p->col_grid);  // persistent: each wave walks several columns
   hipLaunchKernelGGL((k_column<M, MODE, NLK>), dim3(blocks), dim3(256), 0, s, p->spec, p->T, p->g, cd);
 }
 template <int M, int MODE>
@@ -741,7 +760,8 @@ int ash_store_field(ash_plan* p, float* field, void* stream) {
   return check_launch();
 }
 
-int ash_step(ash_plan* p, const ash_step_io* io, void* stream) {
+namespace {
+int check_io(const ash_plan* p, const ash_step_io* io) {
   if (!p || !io) return fail(ASH_E_ARG, "null argument");
   if (!p->loaded) return fail(ASH_E_STATE, "ash_step before ash_load_field");
   if (!(io->dz > 0)) return fail(ASH_E_ARG, "dz must be positive");
@@ -752,17 +772,31 @@ int ash_step(ash_plan* p, const ash_step_io* io, void* stream) {
     return fail(ASH_E_ARG, "at most 4 screens and 4 dense sources per step");
   if (io->src_mask_dev && (!io->src_delay_dev || !io->taxis_dev))
     return fail(ASH_E_ARG, "sparse source needs delay and taxis arrays");
+  return ASH_OK;
+}
+
+// A plain step (no screen, no source, the plan's dz) can replay a captured graph.
+bool plain_step(const ash_plan* p, const ash_step_io* io) {
+  return io->n_screens == 0 && io->n_src_dense == 0 && !io->src_mask_dev && io->dz == p->dz;
+}
+
+// The io fields a captured step depends on.
+bool same_io(const ash_step_io& a, const ash_step_io& b) {
+  return a.pnp_dev == b.pnp_dev && a.ppp_dev == b.ppp_dev && a.pI_dev == b.pI_dev && a.pIloss_dev == b.pIloss_dev &&
+         a.pax_dev == b.pax_dev && a.stats_dev == b.stats_dev && a.max_rows == b.max_rows &&
+         a.field_out_dev == b.field_out_dev && a.dz == b.dz;
+}
+}  // namespace
+
+int ash_step(ash_plan* p, const ash_step_io* io, void* stream) {
+  int rc = check_io(p, io);
+  if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
   if (io->row != p->host_row) {
     hipLaunchKernelGGL(k_set_row, dim3(1), dim3(1), 0, s, p->d_row, io->row);
   }
-  const bool plain = io->n_screens == 0 && io->n_src_dense == 0 && !io->src_mask_dev && io->dz == p->dz;
-  if (plain && p->use_graph) {
-    const bool same = p->gexec && p->gkey.pnp_dev == io->pnp_dev && p->gkey.ppp_dev == io->ppp_dev &&
-                      p->gkey.pI_dev == io->pI_dev && p->gkey.pIloss_dev == io->pIloss_dev &&
-                      p->gkey.pax_dev == io->pax_dev && p->gkey.stats_dev == io->stats_dev &&
-                      p->gkey.max_rows == io->max_rows && p->gkey.field_out_dev == io->field_out_dev &&
-                      p->gkey_dz == io->dz;
+  if (plain_step(p, io) && p->use_graph) {
+    const bool same = p->gexec && same_io(p->gkey, *io);
     if (!same) {
       if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
       if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
@@ -775,13 +809,160 @@ int ash_step(ash_plan* p, const ash_step_io* io, void* stream) {
       p->graph = g;
       HIP_TRY(hipGraphInstantiate(&p->gexec, p->graph, nullptr, nullptr, 0));
       p->gkey = *io;
-      p->gkey_dz = io->dz;
     }
     HIP_TRY(hipGraphLaunch(p->gexec, s));
   } else {
     enqueue_step(p, io, s);
   }
   p->host_row = io->row + 1;
+  return check_launch();
+}
+
+int ash_plan_autotune(ash_plan* p, int32_t* choices, void* stream) {
+  if (!p) return fail(ASH_E_ARG, "null plan");
+  hipStream_t s = (hipStream_t)stream;
+  const size_t nbytes = (size_t)p->g.nX * p->g.nY * p->g.MP * sizeof(float2);
+  float2* save = nullptr;
+  if (p->loaded) {
+    HIP_TRY(hipMalloc(&save, nbytes));
+    HIP_TRY(hipMemcpyAsync(save, p->spec, nbytes, hipMemcpyDeviceToDevice, s));
+  } else {
+    HIP_TRY(hipMemsetAsync(p->spec, 0, nbytes, s));  // finite data for the trial sweeps
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  int rc = ASH_OK;
+  auto timed = [&](auto&& launch) -> float {
+    launch();  // warm-up
+    (void)hipEventRecord(e0, s);
+    for (int r = 0; r < 5; ++r) launch();
+    (void)hipEventRecord(e1, s);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = fail(ASH_E_HIP, "autotune timing failed");
+    return ms / 5.f;
+  };
+  const double dz = p->dz > 0 ? p->dz : p->d.dX;
+  // x-line transforms: forward + inverse per radix order
+  float best = 1e30f;
+  int pick = 0;
+  for (int v = 0; v < 2; ++v) {
+    p->rvx = v;
+    EpiDesc e = epi_base(p);
+    e.half = 1;
+    const float t = timed([&] { xfwd(p, s); xinv(p, 1, e, s); });
+    if (t < best) { best = t; pick = v; }
+  }
+  p->rvx = pick;
+  // y pass per radix order
+  best = 1e30f;
+  pick = 0;
+  const OpDesc op = make_op(p, true, dz / 2, true, dz * 0.5, dz);
+  for (int v = 0; v < 2; ++v) {
+    p->rvy = v;
+    const float t = timed([&] { ypass(p, op, s); });
+    if (t < best) { best = t; pick = v; }
+  }
+  p->rvy = pick;
+  // persistent grid of the column kernels (timed on the nonlinear pass)
+  ColDesc cd = col_base(p);
+  cd.dz = dz;
+  cd.beta_tvd = (float)beta_tvd(p, dz);
+  cd.nl = nl_kind(p);
+  cd.subcycle = 0;
+  cd.taper = 1;
+  cd.mode = COL_NL;
+  best = 1e30f;
+  pick = 2048;
+  for (int g : {512, 1024, 2048, 4096}) {
+    p->col_grid = g;
+    const float t = timed([&] { column(p, cd, s); });
+    if (t < best) { best = t; pick = g; }
+  }
+  p->col_grid = pick;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (save) {
+    HIP_TRY(hipMemcpyAsync(p->spec, save, nbytes, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    (void)hipFree(save);
+  }
+  ++p->epoch;
+  if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
+  if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
+  if (choices) { choices[0] = p->rvx; choices[1] = p->rvy; choices[2] = p->col_grid; }
+  if (rc) return rc;
+  return check_launch();
+}
+
+int ash_plan_set_launch(ash_plan* p, const int32_t* choices) {
+  if (!p || !choices) return fail(ASH_E_ARG, "null argument");
+  if (choices[0] < 0 || choices[0] > 1 || choices[1] < 0 || choices[1] > 1)
+    return fail(ASH_E_ARG, "radix orders are 0 or 1");
+  if (choices[2] < 1 || choices[2] > 65535) return fail(ASH_E_ARG, "column grid out of range");
+  p->rvx = choices[0];
+  p->rvy = choices[1];
+  p->col_grid = choices[2];
+  ++p->epoch;
+  if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
+  if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
+  return ASH_OK;
+}
+
+int ash_batch_create(ash_batch** out) {
+  if (!out) return fail(ASH_E_ARG, "null argument");
+  *out = new ash_batch();
+  return ASH_OK;
+}
+
+void ash_batch_destroy(ash_batch* b) {
+  if (!b) return;
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->cap) (void)hipStreamDestroy(b->cap);
+  delete b;
+}
+
+int ash_batch_step(ash_batch* b, ash_plan* const* plans, const ash_step_io* ios, int32_t n, void* stream) {
+  if (!b || !plans || !ios || n < 1) return fail(ASH_E_ARG, "bad argument");
+  bool all_plain = true;
+  for (int i = 0; i < n; ++i) {
+    int rc = check_io(plans[i], &ios[i]);
+    if (rc) return rc;
+    for (int j = 0; j < i; ++j)
+      if (plans[j] == plans[i]) return fail(ASH_E_ARG, "a plan appears twice in the batch");
+    all_plain = all_plain && plain_step(plans[i], &ios[i]);
+  }
+  hipStream_t s = (hipStream_t)stream;
+  for (int i = 0; i < n; ++i)
+    if (ios[i].row != plans[i]->host_row)
+      hipLaunchKernelGGL(k_set_row, dim3(1), dim3(1), 0, s, plans[i]->d_row, ios[i].row);
+  if (all_plain) {
+    bool same = b->gexec && (int)b->plans.size() == n;
+    for (int i = 0; same && i < n; ++i)
+      same = b->plans[i] == plans[i] && b->epochs[i] == plans[i]->epoch && same_io(b->keys[i], ios[i]);
+    if (!same) {
+      if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+      if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+      if (!b->cap) HIP_TRY(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
+      HIP_TRY(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+      for (int i = 0; i < n; ++i) enqueue_step(plans[i], &ios[i], b->cap);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(b->cap, &g);
+      if (e != hipSuccess) return fail(ASH_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+      b->graph = g;
+      HIP_TRY(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+      b->plans.assign(plans, plans + n);
+      b->keys.assign(ios, ios + n);
+      b->epochs.resize(n);
+      for (int i = 0; i < n; ++i) b->epochs[i] = plans[i]->epoch;
+    }
+    HIP_TRY(hipGraphLaunch(b->gexec, s));
+  } else {
+    for (int i = 0; i < n; ++i) enqueue_step(plans[i], &ios[i], s);
+  }
+  for (int i = 0; i < n; ++i) plans[i]->host_row = ios[i].row + 1;
   return check_launch();
 }
 
@@ -1103,7 +1284,8 @@ int ash_plan_set_learned(ash_plan* p, ash_net* n, double dpdt_max) {
   }
   p->net = n;
   p->dpdt_max = dpdt_max;
-  // the captured step graph holds the other update
+  // the captured step graphs hold the other update
+  ++p->epoch;
   if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
   if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
   return ASH_OK;

@@ -146,19 +146,25 @@ __device__ __forceinline__ void dft(float2* v) {
 
 // ---------------------------------------------------------------------------
 // Pass lists.  Forward order R0, R1, [R2]; the inverse runs them reversed.
+// Two radix orders per line length (V = 0, 1); a plan's autotuner picks one.
 // ---------------------------------------------------------------------------
-template <int L> struct Radix;
-template <> struct Radix<64>   { static constexpr int NP = 2; static constexpr int R0 = 8,  R1 = 8,  R2 = 1; };
-template <> struct Radix<128>  { static constexpr int NP = 2; static constexpr int R0 = 16, R1 = 8,  R2 = 1; };
-template <> struct Radix<192>  { static constexpr int NP = 3; static constexpr int R0 = 16, R1 = 4,  R2 = 3; };
-template <> struct Radix<256>  { static constexpr int NP = 2; static constexpr int R0 = 16, R1 = 16, R2 = 1; };
-template <> struct Radix<384>  { static constexpr int NP = 3; static constexpr int R0 = 16, R1 = 8,  R2 = 3; };
-template <> struct Radix<512>  { static constexpr int NP = 3; static constexpr int R0 = 8,  R1 = 8,  R2 = 8; };
-template <> struct Radix<1024> { static constexpr int NP = 3; static constexpr int R0 = 16, R1 = 8,  R2 = 8; };
+template <int L, int V = 0> struct Radix;
+template <> struct Radix<64, 0>   { static constexpr int NP = 2; static constexpr int R0 = 8,  R1 = 8,  R2 = 1; };
+template <> struct Radix<64, 1>   { static constexpr int NP = 2; static constexpr int R0 = 16, R1 = 4,  R2 = 1; };
+template <> struct Radix<128, 0>  { static constexpr int NP = 2; static constexpr int R0 = 16, R1 = 8,  R2 = 1; };
+template <> struct Radix<128, 1>  { static constexpr int NP = 3; static constexpr int R0 = 8,  R1 = 4,  R2 = 4; };
+template <> struct Radix<192, 0>  { static constexpr int NP = 3; static constexpr int R0 = 16, R1 = 4,  R2 = 3; };
+template <> struct Radix<192, 1>  { static constexpr int NP = 3; static constexpr int R0 = 8,  R1 = 8,  R2 = 3; };
+template <> struct Radix<256, 0>  { static constexpr int NP = 2; static constexpr int R0 = 16, R1 = 16, R2 = 1; };
+template <> struct Radix<256, 1>  { static constexpr int NP = 3; static constexpr int R0 = 8,  R1 = 8,  R2 = 4; };
+template <> struct Radix<384, 0>  { static constexpr int NP = 3; static constexpr int R0 = 16, R1 = 8,  R2 = 3; };
+template <> struct Radix<384, 1>  { static constexpr int NP = 3; static constexpr int R0 = 8,  R1 = 16, R2 = 3; };
+template <> struct Radix<512, 0>  { static constexpr int NP = 3; static constexpr int R0 = 8,  R1 = 8,  R2 = 8; };
+template <> struct Radix<1024, 0> { static constexpr int NP = 3; static constexpr int R0 = 16, R1 = 8,  R2 = 8; };
 
-template <int L, int P, bool INV>
+template <int L, int P, bool INV, int V = 0>
 struct PassR {
-  using RX = Radix<L>;
+  using RX = Radix<L, V>;
   static constexpr int NP = RX::NP;
   // forward pass p uses R_p; inverse pass p uses R_{NP-1-p}
   static constexpr int idx = INV ? (NP - 1 - P) : P;
@@ -188,9 +194,9 @@ __device__ __forceinline__ void twiddle(float2* v, int j, const float2* tw) {
   }
 }
 
-template <int L, int P, bool INV>
+template <int L, int P, bool INV, int V = 0>
 __device__ __forceinline__ int out_pos(int j, int r) {
-  using PR = PassR<L, P, INV>;
+  using PR = PassR<L, P, INV, V>;
   constexpr int Ns = PR::Ns, R = PR::R;
   return (j / Ns) * Ns * R + (j % Ns) + r * Ns;
 }

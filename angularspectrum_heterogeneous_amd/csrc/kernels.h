@@ -39,10 +39,10 @@ struct Lay {
 // One Stockham pass.  IN: 0 = load functor, 1 = LDS, 2 = registers rg;
 // OUT: 0 = store functor, 1 = LDS, 2 = registers rg ([q * R + r] of the
 // butterflies this thread owns).
-template <int L, int P, bool INV, int NT, int NL, class LY, int IN, int OUT, class LoadF, class StoreF>
+template <int L, int P, bool INV, int NT, int NL, class LY, int IN, int OUT, int V, class LoadF, class StoreF>
 __device__ __forceinline__ void fft_pass_io(float2* buf, const float2* tw, int tid, LoadF& load, StoreF& store,
                                             float2* rg) {
-  using PR = PassR<L, P, INV>;
+  using PR = PassR<L, P, INV, V>;
   constexpr int R = PR::R, NB = PR::NB, Ns = PR::Ns;
   constexpr int TOT = NB * NL;
   constexpr int PER = (TOT + NT - 1) / NT;
@@ -72,7 +72,7 @@ __device__ __forceinline__ void fft_pass_io(float2* buf, const float2* tw, int t
       LY::template map<NB>(b, line, j);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const int pos = out_pos<L, P, INV>(j, r);
+        const int pos = out_pos<L, P, INV, V>(j, r);
         if constexpr (OUT == 0) store(line, pos, v[q][r]);
         else if constexpr (OUT == 1) buf[LY::idx(line, pos)] = v[q][r];
         else rg[q * R + r] = v[q][r];
@@ -82,18 +82,19 @@ __device__ __forceinline__ void fft_pass_io(float2* buf, const float2* tw, int t
   if constexpr (OUT == 1) __syncthreads();
 }
 
-template <int L, int P, bool INV, int NT, int NL, class LY, class LoadF, class StoreF>
+template <int L, int P, bool INV, int NT, int NL, class LY, int V, class LoadF, class StoreF>
 __device__ __forceinline__ void fft_pass(float2* buf, const float2* tw, int tid, LoadF& load, StoreF& store) {
-  constexpr int NP = PassR<L, P, INV>::NP;
-  fft_pass_io<L, P, INV, NT, NL, LY, P == 0 ? 0 : 1, P == NP - 1 ? 0 : 1>(buf, tw, tid, load, store, nullptr);
+  constexpr int NP = PassR<L, P, INV, V>::NP;
+  fft_pass_io<L, P, INV, NT, NL, LY, P == 0 ? 0 : 1, P == NP - 1 ? 0 : 1, V>(buf, tw, tid, load, store, nullptr);
 }
 
-template <int L, bool INV, int NT, int NL, class LY, class LoadF, class StoreF>
+// V selects the radix order (Radix<L, V>).
+template <int L, bool INV, int NT, int NL, class LY, int V = 0, class LoadF, class StoreF>
 __device__ __forceinline__ void fft_run(float2* buf, const float2* tw, int tid, LoadF load, StoreF store) {
-  constexpr int NP = Radix<L>::NP;
-  fft_pass<L, 0, INV, NT, NL, LY>(buf, tw, tid, load, store);
-  if constexpr (NP > 1) fft_pass<L, 1, INV, NT, NL, LY>(buf, tw, tid, load, store);
-  if constexpr (NP > 2) fft_pass<L, 2, INV, NT, NL, LY>(buf, tw, tid, load, store);
+  constexpr int NP = Radix<L, V>::NP;
+  fft_pass<L, 0, INV, NT, NL, LY, V>(buf, tw, tid, load, store);
+  if constexpr (NP > 1) fft_pass<L, 1, INV, NT, NL, LY, V>(buf, tw, tid, load, store);
+  if constexpr (NP > 2) fft_pass<L, 2, INV, NT, NL, LY, V>(buf, tw, tid, load, store);
 }
 
 template <int L>
@@ -265,7 +266,7 @@ __device__ __forceinline__ float2 visit_operator(const OpDesc& op, double kk, co
 // ---------------------------------------------------------------------------
 // x-sweep, forward FFT along x.  Tile = (y, f-block).
 // ---------------------------------------------------------------------------
-template <int L>
+template <int L, int V>
 __global__ __launch_bounds__(LNT) void k_xfft_fwd(float2* __restrict__ spec, Tables T, Geom g) {
   __shared__ float2 tw[L];
   __shared__ float2 buf[L * LF];
@@ -284,7 +285,7 @@ __global__ __launch_bounds__(LNT) void k_xfft_fwd(float2* __restrict__ spec, Tab
   auto st = [&](int line, int pos, float2 v) {
     if (line < nvalid) base[pos * sx + line] = v;
   };
-  fft_run<L, false, LNT, LF, LY>(buf, tw, tid, ld, st);
+  fft_run<L, false, LNT, LF, LY, V>(buf, tw, tid, ld, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -298,7 +299,7 @@ __global__ __launch_bounds__(LNT) void k_xfft_fwd(float2* __restrict__ spec, Tab
 // only through kk = kx^2 + ky^2, so one block evaluates it once per
 // (|ky| index, f bin) into an LDS table and applies it to both rows: a
 // quarter of one evaluation per element.
-template <int L, bool PRECISE>
+template <int L, bool PRECISE, int V>
 __global__ __launch_bounds__(LNT) void k_ypass(float2* __restrict__ spec, Tables T, Geom g, OpDesc op) {
   constexpr int H = L / 2;
   __shared__ float2 tw[L];
@@ -358,7 +359,7 @@ __global__ __launch_bounds__(LNT) void k_ypass(float2* __restrict__ spec, Tables
     auto st_lds = [&](int line, int pos, float2 v) { buf[LY::idx(line, pos)] = v; };
     // pass 0 stores to LDS only after its barrier, which follows every read of
     // the previous row's last pass
-    fft_run<L, false, LNT, LF, LY>(buf, tw, t, ld, st_lds);
+    fft_run<L, false, LNT, LF, LY, V>(buf, tw, t, ld, st_lds);
     auto ld_op = [&](int line, int pos) -> float2 {
       const int i = pos < H ? pos : L - 1 - pos;
       return cmul(buf[LY::idx(line, pos)], opt[i * LF + line]);
@@ -367,7 +368,7 @@ __global__ __launch_bounds__(LNT) void k_ypass(float2* __restrict__ spec, Tables
       if (line < nvalid) base[pos * sy + line] = v;
     };
     __syncthreads();
-    fft_run<L, true, LNT, LF, LY>(buf, tw, t, ld_op, st);
+    fft_run<L, true, LNT, LF, LY, V>(buf, tw, t, ld_op, st);
   }
 }
 
@@ -386,7 +387,7 @@ struct EpiDesc {
   float scale;          // EPI 2: factor on the transformed values (1/nX when no y-pass normalised them)
 };
 
-template <int L, int EPI>
+template <int L, int EPI, int V>
 __global__ __launch_bounds__(LNT) void k_xfft_inv(float2* __restrict__ spec, Tables T, Geom g, EpiDesc e) {
   __shared__ float2 tw[L];
   __shared__ float2 buf[L * LF];
@@ -413,7 +414,7 @@ __global__ __launch_bounds__(LNT) void k_xfft_inv(float2* __restrict__ spec, Tab
       if constexpr (EPI == 1) v = cscale(v, tx[pos] * ty);
       if (line < nvalid) base[pos * sx + line] = v;
     };
-    fft_run<L, true, LNT, LF, LY>(buf, tw, tid, ld, st);
+    fft_run<L, true, LNT, LF, LY, V>(buf, tw, tid, ld, st);
   } else {
     // the inverse transform's last pass stores into LDS with the scale, the
     // DC/Nyquist projection and the frequency-weighted mask already applied,
@@ -441,7 +442,7 @@ __global__ __launch_bounds__(LNT) void k_xfft_inv(float2* __restrict__ spec, Tab
       if (e.fw) v = cscale(v, exp2f(sfwe[line] * (lgx[pos] + lgy)));
       buf[LY::idx(line, pos)] = v;
     };
-    fft_run<L, true, LNT, LF, LY>(buf, tw, tid, ld, st_lds);
+    fft_run<L, true, LNT, LF, LY, V>(buf, tw, tid, ld, st_lds);
     __syncthreads();
     // element e -> (line, pos), line fastest: coalesced 128-byte rows; the
     // line of a thread is fixed (LNT is a multiple of LF)

@@ -73,6 +73,18 @@ class MarchPlan:
         check(self.lib.ash_plan_set_learned(self.h, net.h if net is not None else None, float(dpdt_max)),
               "ash_plan_set_learned")
 
+    def autotune(self) -> dict:
+        """Times the radix orders and the column grid on this plan and keeps the fastest."""
+        ch = (ctypes.c_int32 * 3)()
+        check(self.lib.ash_plan_autotune(self.h, ch, _lib.current_stream()), "ash_plan_autotune")
+        self.tuned = {"x_radix_order": int(ch[0]), "y_radix_order": int(ch[1]), "column_grid": int(ch[2])}
+        return self.tuned
+
+    def set_launch(self, x_radix_order: int = 0, y_radix_order: int = 0, column_grid: int = 2048) -> None:
+        """Pins the launch configuration (the autotuner's choices)."""
+        ch = (ctypes.c_int32 * 3)(x_radix_order, y_radix_order, column_grid)
+        check(self.lib.ash_plan_set_launch(self.h, ch), "ash_plan_set_launch")
+
     def set_dz(self, dz: float) -> None:
         check(self.lib.ash_plan_set_dz(self.h, float(dz)), "ash_plan_set_dz")
         self.dz = float(dz)

@@ -153,70 +153,84 @@ def _upload_screens(screens):
     return out
 
 
-def _march(plan: MarchPlan, field0, p: SolverParams, N, dZ, verbose, cb, screens, check_every):
-    nX, nY, nT = field0.shape
-    dev = field0.device
-    plan.set_dz(dZ)
-    max_steps = int(np.ceil(p.propDist / dZ)) + 100
-    outs = torch.zeros((4, max_steps, nX, nY), dtype=torch.float32, device=dev)
-    pax = torch.zeros((max_steps, nT), dtype=torch.float32, device=dev)
-    stats = torch.zeros(max_steps + 1, dtype=torch.int32, device=dev)
-    field_cb = torch.empty((nX, nY, nT), dtype=torch.float32, device=dev) if cb is not None else None
-    plan.load(field0, stats)
+class _Marcher:
+    """One solve's z loop as a state machine (ASS:1413-1550): ``prepare`` fills the
+    step io (screens and sources crossed in the step), the caller launches the
+    step, ``finish`` books it (deferred stability check every ``check_every``
+    steps, callback) and ``result`` returns the outputs.  Used by the single
+    solve and by the batched solve."""
 
-    io = AshStepIO()
-    io.max_rows = max_steps
-    io.pnp_dev, io.ppp_dev, io.pI_dev, io.pIloss_dev = (outs[k].data_ptr() for k in range(4))
-    io.pax_dev = pax.data_ptr()
-    io.stats_dev = stats.data_ptr()
-    io.field_out_dev = field_cb.data_ptr() if field_cb is not None else None
+    def __init__(self, plan: MarchPlan, field0, p: SolverParams, N, dZ, verbose, cb, screens, check_every):
+        nX, nY, nT = field0.shape
+        self.plan, self.p, self.N, self.dZ = plan, p, N, dZ
+        self.verbose, self.cb, self.screens, self.check_every = verbose, cb, screens, check_every
+        self.dev = dev = field0.device
+        plan.set_dz(dZ)
+        self.max_steps = max_steps = int(np.ceil(p.propDist / dZ)) + 100
+        self.outs = torch.zeros((4, max_steps, nX, nY), dtype=torch.float32, device=dev)
+        self.pax = torch.zeros((max_steps, nT), dtype=torch.float32, device=dev)
+        self.stats = torch.zeros(max_steps + 1, dtype=torch.int32, device=dev)
+        self.field_cb = torch.empty((nX, nY, nT), dtype=torch.float32, device=dev) if cb is not None else None
+        plan.load(field0, self.stats)
+        io = self.io = AshStepIO()
+        io.max_rows = max_steps
+        io.pnp_dev, io.ppp_dev, io.pI_dev, io.pIloss_dev = (self.outs[k].data_ptr() for k in range(4))
+        io.pax_dev = self.pax.data_ptr()
+        io.stats_dev = self.stats.data_ptr()
+        io.field_out_dev = self.field_cb.data_ptr() if self.field_cb is not None else None
+        self.keep = []      # device copies of sources crossed in the current step
+        self.zvec = []
+        self.z = 0.0        # running sum(zvec) (ASS:1413-1415 recomputes it every step)
+        self.dz_steps = []  # dZ_step of each step (stability check)
+        self.checked = 0
+        self.cc = 0
 
-    keep = []          # device copies of sources crossed in the current step
-    zvec = []
-    dz_steps = []      # dZ_step of each step (stability check)
-    checked = 0
-    cc = 0
+    @property
+    def done(self) -> bool:
+        return self.z >= self.p.propDist - 1e-15
 
-    def check_stability(upto):
-        nonlocal checked
-        if upto <= checked:
+    def _check_stability(self, upto):
+        if upto <= self.checked:
             return
-        mx = stats[checked:upto].cpu().view(torch.float32).numpy().astype(np.float64)
-        for k in range(upto - checked):
-            step = checked + k
-            if N * dz_steps[step] / p.dT * mx[k] > p.stabilityThreshold:
+        p, N = self.p, self.N
+        mx = self.stats[self.checked:upto].cpu().view(torch.float32).numpy().astype(np.float64)
+        for k in range(upto - self.checked):
+            step = self.checked + k
+            if N * self.dz_steps[step] / p.dT * mx[k] > p.stabilityThreshold:
                 new_dz = max(p.stabilityRecoveryFactor * p.dT / (mx[k] * N), p.dZmin)
-                if new_dz == dZ:
+                if new_dz == self.dZ:
                     raise StabilityError(
                         f'stability criterion violated at step {step} (max|p|={mx[k]:.4g}) and dZ '
                         f'cannot shrink below dZmin={p.dZmin}')
                 raise _Restart(new_dz)
-        checked = upto
+        self.checked = upto
 
-    while sum(zvec) < p.propDist - 1e-15:
-        if cc >= max_steps:
+    def prepare(self) -> AshStepIO:
+        p, io, cc = self.p, self.io, self.cc
+        if cc >= self.max_steps:
             raise RuntimeError('step count exceeded the output allocation')
-        z_cur = sum(zvec)
+        z_cur = self.z
         remaining = p.propDist - z_cur
-        dZ_step = min(dZ, remaining)
-        dz_steps.append(dZ_step)
-        if verbose:
+        dZ_step = min(self.dZ, remaining)
+        self.dz_steps.append(dZ_step)
+        if self.verbose:
             print(f'z = {z_cur + dZ_step:.6f} m  (step {cc})')
         io.row = cc
         io.dz = dZ_step
         # screens crossed in this step (ASS:1505-1518)
         io.n_screens = 0
-        if screens is not None:
-            for z_ps, ph, amp in screens:
+        if self.screens is not None:
+            for z_ps, ph, amp in self.screens:
                 if z_cur < z_ps <= z_cur + dZ_step:
                     if io.n_screens == 4:
                         raise NotImplementedError('more than 4 screens in one step')
                     io.screen_phase_dev[io.n_screens] = ph.data_ptr()
                     io.screen_amp_dev[io.n_screens] = amp.data_ptr() if amp is not None else None
                     io.n_screens += 1
-                    if verbose:
+                    if self.verbose:
                         print(f'  applied phase screen at z = {z_ps:.4f} m')
         # sources crossed in this step (ASS:1521-1527)
+        keep = self.keep
         keep.clear()
         io.n_src_dense = 0
         io.src_mask_dev = None
@@ -233,31 +247,46 @@ def _march(plan: MarchPlan, field0, p: SolverParams, N, dZ, verbose, cb, screens
                         keep.append(t)
                         io.src_dense_dev[io.n_src_dense] = t.data_ptr()
                         io.n_src_dense += 1
-                    if verbose:
+                    if self.verbose:
                         print(f'  injected source plane at z = {sp_z*1e3:.2f} mm')
             if sparse:
                 merged = SourceSlice.merge(sparse)
-                mask, delay, tax = merged.device_arrays(dev)
+                mask, delay, tax = merged.device_arrays(self.dev)
                 keep.extend([mask, delay, tax])
                 io.src_mask_dev = mask.data_ptr()
                 io.src_delay_dev = delay.data_ptr()
                 io.taxis_dev = tax.data_ptr()
                 io.src_f0, io.src_p0 = merged.f0, merged.p0
                 io.src_ncycles, io.src_dur = merged.ncycles, float(merged.dur)
-        plan.step(io)
-        zvec.append(dZ_step)
-        if keep:
-            torch.cuda.current_stream().synchronize()
-        if cb is not None:
-            check_stability(cc + 1)
-            cb(cc, sum(zvec), field_cb)
-        cc += 1
-        if cc % check_every == 0:
-            check_stability(cc)
-    check_stability(cc)
+        return io
 
-    field = plan.store()
-    res = outs[:, :cc].permute(0, 2, 3, 1).contiguous().cpu().numpy()
-    pax_np = pax[:cc].t().contiguous().cpu().numpy()
-    zaxis = np.cumsum(zvec[:cc])
-    return field.cpu().numpy(), res[0], res[1], res[2], res[3], zaxis, pax_np
+    def finish(self) -> None:
+        dZ_step = self.dz_steps[-1]
+        self.zvec.append(dZ_step)
+        self.z = sum(self.zvec)
+        if self.keep:
+            torch.cuda.current_stream().synchronize()
+        if self.cb is not None:
+            self._check_stability(self.cc + 1)
+            self.cb(self.cc, self.z, self.field_cb)
+        self.cc += 1
+        if self.cc % self.check_every == 0:
+            self._check_stability(self.cc)
+
+    def result(self):
+        cc = self.cc
+        self._check_stability(cc)
+        field = self.plan.store()
+        res = self.outs[:, :cc].permute(0, 2, 3, 1).contiguous().cpu().numpy()
+        pax_np = self.pax[:cc].t().contiguous().cpu().numpy()
+        zaxis = np.cumsum(self.zvec[:cc])
+        return field.cpu().numpy(), res[0], res[1], res[2], res[3], zaxis, pax_np
+
+
+def _march(plan: MarchPlan, field0, p: SolverParams, N, dZ, verbose, cb, screens, check_every):
+    m = _Marcher(plan, field0, p, N, dZ, verbose, cb, screens, check_every)
+    while not m.done:
+        io = m.prepare()
+        plan.step(io)
+        m.finish()
+    return m.result()

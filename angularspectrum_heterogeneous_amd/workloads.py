@@ -93,5 +93,31 @@ def config3(**over):
     return planes[0][1], kw
 
 
+def config4(n=32, seed=0, propDist=0.02, sizes=(128, 192, 256, 384), lengths=(512, 1024, 2048)):
+    """Batched transmit configurations: seeded focused pistons of uneven size.
+
+    Instance k draws nX = nY from ``sizes`` and nT from ``lengths``, a focus
+    depth uniform in 30-70 mm, a lateral (x) steering of the focal point
+    uniform in +-10 mm and p0 uniform in 0.2-1.5 MPa; 1 MHz, 5 cycles, piston
+    radius 0.3 nX dX, the boundaries of the 256x256x1024 case, dZ = dX.
+    Returns [(field, kwargs, meta)] with meta the drawn values."""
+    rng = np.random.default_rng(seed)
+    dX, dT = 3.08e-4, 4e-8
+    out = []
+    for _ in range(n):
+        nx = int(rng.choice(sizes))
+        nt = int(rng.choice(lengths))
+        focus = float(rng.uniform(30e-3, 70e-3))
+        steer = float(rng.uniform(-10e-3, 10e-3))
+        p0 = float(rng.uniform(0.2e6, 1.5e6))
+        radius = 0.3 * nx * dX
+        field = focused_piston(nX=nx, nY=nx, nT=nt, dX=dX, dT=dT, p0=p0, radius=radius, focus=focus,
+                               steer=(steer, 0.0))
+        kw = params_kwargs(dX=dX, dY=dX, dT=dT, propDist=propDist, dZmin=dX, boundaryFactor=0.1,
+                           useFreqWeightedBoundary=True, useSuperAbsorbing=True)
+        out.append((field, kw, dict(nX=nx, nT=nt, focus=focus, steer=steer, p0=p0, radius=radius)))
+    return out
+
+
 def voxels(shape):
     return int(np.prod(shape))

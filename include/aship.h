@@ -187,6 +187,31 @@ int ash_apply_phase_screen(ash_plan* plan, const float* in_dev, float* out_dev, 
 #define ASH_K_NET 8         /* learned nonlinear step (network inference) */
 #define ASH_K_COUNT 9
 
+/* Launch-configuration autotuner, no counterpart in the reference: times the
+ * two radix orders of the x-line and of the y-line transforms and four
+ * persistent grid sizes of the column kernels on this plan, and keeps the
+ * fastest of each.  A loaded field is saved and restored; before
+ * ash_load_field the spectrum is cleared first.  The radix order changes the
+ * float32 rounding of the transforms (not their result).  choices (optional,
+ * 3 entries): x radix order, y radix order, column grid. */
+int ash_plan_autotune(ash_plan* plan, int32_t* choices, void* stream);
+
+/* Pins the launch configuration (same 3 entries as ash_plan_autotune reports),
+ * e.g. to repeat a tuned run bit for bit.  Defaults: 0, 0, 2048. */
+int ash_plan_set_launch(ash_plan* plan, const int32_t* choices);
+
+/* Batched march: one z-step of n independent plans (plan i with ios[i]), for
+ * solves of many transmit configurations.  No counterpart in the reference,
+ * whose solver runs one configuration per call (ASS:1217).  When every step
+ * is plain (as for ash_step), the steps of all plans are captured into one
+ * hipGraph and one graph launch advances the whole batch; the graph is
+ * replayed while the plan set and the io buffers stay the same.  Plans may
+ * have different grid sizes. */
+typedef struct ash_batch ash_batch;
+int ash_batch_create(ash_batch** out);
+void ash_batch_destroy(ash_batch* batch);
+int ash_batch_step(ash_batch* batch, ash_plan* const* plans, const ash_step_io* ios, int32_t n, void* stream);
+
 /* Timing aid with no counterpart in the reference (whose only timing is the
  * wall clock of ASS:1411, :1552): runs n_iter steps like ash_step, without the
  * graph, with HIP events recorded on `stream` around every kernel launch.

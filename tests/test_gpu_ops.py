@@ -118,6 +118,25 @@ def test_march_step_split_kt(shape):
     assert e < 5e-6
 
 
+@pytest.mark.parametrize("shape", SIZES)
+@pytest.mark.parametrize("launch", [(1, 0, 2048), (0, 1, 512), (1, 1, 4096)])
+def test_march_step_launch_configs(shape, launch):
+    """The autotuner's alternatives (radix orders of the x and y transforms, column grid)."""
+    p = _params(shape)
+    nX, nY, nT = shape
+    dz = 3.08e-4
+    u = _field(shape, p0=1e6)
+    N = p.beta / (2 * p.c0 ** 3 * p.rho0)
+    plan = _plan(shape, p, dz)
+    plan.set_launch(*launch)
+    got = plan.march_step(_dev(u)).cpu().numpy()
+    ops = O.build_operators(nX, nY, nT, p, dz, np.float64)
+    ref = O.march_step(u.astype(np.float64), ops, p, N, dz)
+    e = rel_l2(got, ref)
+    print(shape, launch, e)
+    assert e < 5e-6
+
+
 @pytest.mark.parametrize("shape", SIZES[:2])
 def test_freq_weighted_boundary(shape):
     p = _params(shape, useFreqWeightedBoundary=True)
