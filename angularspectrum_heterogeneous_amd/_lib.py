@@ -37,6 +37,7 @@ class AshDesc(ctypes.Structure):
         ("superAbsorbingStrength", c_dbl),
         ("fluxScheme", c_i32), ("useObliquityCorrection", c_i32),
         ("subcycle", c_i32), ("cflTarget", c_dbl),
+        ("useNonlinearityObliquity", c_i32),
     ]
 
 

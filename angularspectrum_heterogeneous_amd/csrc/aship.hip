@@ -91,6 +91,7 @@ struct ash_plan {
   double* ibefore = nullptr;
   int* d_row = nullptr;
   unsigned* scratch_max = nullptr;
+  double* obl_acc = nullptr;  // [2] power-weighted obliquity sums of the current step
   Tables T{};
   std::vector<void*> allocs;
   int64_t ws_bytes = 0;
@@ -475,6 +476,16 @@ void xinv(ash_plan* p, int epi, const EpiDesc& e, hipStream_t s) {
   else if (epi == 1) { DISPATCH_L(p->g.nX, (launch_xinv_L<L, 1>(p, e, s))); }
   else { DISPATCH_L(p->g.nX, (launch_xinv_L<L, 2>(p, e, s))); }
 }
+template <int L>
+void launch_ypower_L(ash_plan* p, hipStream_t s) {
+  const dim3 grid(p->g.nX * p->g.nfb);
+  if (p->rvy) hipLaunchKernelGGL((k_ypower<L, 1>), grid, dim3(LNT), 0, s, p->spec, p->T, p->g, p->obl_acc);
+  else hipLaunchKernelGGL((k_ypower<L, 0>), grid, dim3(LNT), 0, s, p->spec, p->T, p->g, p->obl_acc);
+}
+void ypower(ash_plan* p, hipStream_t s) {
+  ProfScope ps(p, ASH_K_YPASS, s);
+  DISPATCH_L(p->g.nY, launch_ypower_L<L>(p, s));
+}
 void ysa(ash_plan* p, const ScreenDesc& sd, hipStream_t s) {
   ProfScope ps(p, ASH_K_YSA, s);
   DISPATCH_L(p->g.nY, launch_ysa_L<L>(p, sd, s));
@@ -563,6 +574,26 @@ void learned_nl(ash_plan* p, ColDesc cd, double dz, hipStream_t s) {
   column(p, cd, s);
 }
 
+// Beam-averaged nonlinear obliquity (ASS:923-965): the absorbing-layer taper in
+// t is applied here (the reference scales N on the field after abl), then the
+// power-weighted <k/k_z> of the field's 3-D spectrum is summed into obl_acc by
+// an x transform, a read-only y transform and the inverse x transform; the
+// nonlinear column pass that follows reads N <k/k_z> from obl_acc.
+void nl_obliquity(ash_plan* p, ColDesc& cd, hipStream_t s) {
+  ColDesc c = cd;
+  c.mode = COL_TAPER;
+  c.nl = NL_NONE;
+  column(p, c, s);
+  hipLaunchKernelGGL(k_zero2, dim3(1), dim3(1), 0, s, p->obl_acc);
+  xfwd(p, s);
+  ypower(p, s);
+  EpiDesc e = epi_base(p);
+  e.scale = 1.f / (float)p->g.nX;
+  xinv(p, 2, e, s);  // fw = sa = 0: scale and DC/Nyquist projection only
+  cd.taper = 0;
+  cd.nscale = p->obl_acc;
+}
+
 // The composite march step (ASS:874-913) on the spectral state, leaving the
 // state after the last attenuation visit with epilogue `epi_last` fused.
 void enqueue_composite(ash_plan* p, double dz, int epi_last, const EpiDesc& elast, hipStream_t s) {
@@ -578,7 +609,8 @@ void enqueue_composite(ash_plan* p, double dz, int epi_last, const EpiDesc& elas
     ypass(p, make_op(p, true, dz / 2, true, dz * 0.5, dz), s);
     e.half = 1;
     xinv(p, 1, e, s);
-    cd.taper = 1;  // also the taper of the COL_TAPER pass below
+    cd.taper = 1;
+    if (p->d.useNonlinearityObliquity) nl_obliquity(p, cd, s);
     if (p->net) {
       learned_nl(p, cd, dz, s);
     } else {
@@ -590,6 +622,8 @@ void enqueue_composite(ash_plan* p, double dz, int epi_last, const EpiDesc& elas
     xinv(p, 1, e, s);
     cd.mode = COL_TAPER;
     cd.nl = NL_NONE;
+    cd.taper = 1;
+    cd.nscale = nullptr;
     column(p, cd, s);
     xfwd(p, s);
     ypass(p, make_op(p, false, 0.0, true, dz * 0.5, dz), s);
@@ -599,8 +633,9 @@ void enqueue_composite(ash_plan* p, double dz, int epi_last, const EpiDesc& elas
     ypass(p, make_op(p, true, dz, false, 0.0, dz), s);
     e.half = 0;
     xinv(p, 1, e, s);
-    cd.mode = COL_NL;
     cd.taper = 2;
+    if (p->d.useNonlinearityObliquity) nl_obliquity(p, cd, s);
+    cd.mode = COL_NL;
     column(p, cd, s);
     xfwd(p, s);
     ypass(p, make_op(p, false, 0.0, true, dz * 1.0, dz), s);
@@ -709,7 +744,8 @@ int ash_plan_create(const ash_desc* desc, ash_plan** out) {
   hipError_t e;
   if ((e = hipMalloc(&p->spec, nspec * sizeof(float2))) != hipSuccess ||
       (e = hipMalloc(&p->ibefore, (size_t)d.nX * d.nY * sizeof(double))) != hipSuccess ||
-      (e = hipMalloc(&p->d_row, 64)) != hipSuccess || (e = hipMalloc(&p->scratch_max, 64)) != hipSuccess) {
+      (e = hipMalloc(&p->d_row, 64)) != hipSuccess || (e = hipMalloc(&p->scratch_max, 64)) != hipSuccess ||
+      (e = hipMalloc(&p->obl_acc, 64)) != hipSuccess) {
     ash_plan_destroy(p);
     return fail(ASH_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
   }
@@ -717,6 +753,7 @@ int ash_plan_create(const ash_desc* desc, ash_plan** out) {
   p->allocs.push_back(p->ibefore);
   p->allocs.push_back(p->d_row);
   p->allocs.push_back(p->scratch_max);
+  p->allocs.push_back(p->obl_acc);
   // padding bins are read (masked) but never written: clear once
   if ((e = hipMemset(p->spec, 0, nspec * sizeof(float2))) != hipSuccess ||
       (e = hipMemset(p->d_row, 0, 64)) != hipSuccess || (e = hipDeviceSynchronize()) != hipSuccess) {
@@ -1271,6 +1308,8 @@ int ash_plan_set_learned(ash_plan* p, ash_net* n, double dpdt_max) {
   if (n) {
     if (!p->d.useSplitStep || p->d.fluxScheme != 1)
       return fail(ASH_E_UNSUPPORTED, "the learned step replaces the split-step KT update only");
+    if (p->d.useNonlinearityObliquity)
+      return fail(ASH_E_UNSUPPORTED, "the learned step does not take the beam obliquity scale");
     if (n->nT != p->d.nT) return fail(ASH_E_ARG, "network column length differs from the plan's nT");
     const size_t ncol = (size_t)p->d.nX * p->d.nY;
     if (!p->rscratch) {

@@ -171,6 +171,8 @@ __global__ __launch_bounds__(256) void k_column(float2* __restrict__ spec, Table
   constexpr bool rewrite = (MODE == COL_NL || MODE == COL_TAPER || MODE == COL_LOAD || MODE == COL_FINAL_SRC);
   const int row = FIN ? *d.row : 0;
   const float* tt = d.taper == 1 ? T.tapt_h : T.tapt_f;
+  // Burgers coefficient, scaled by the beam obliquity when requested (ASS:934)
+  const double Nn = d.nscale ? d.N * (d.nscale[0] / (d.nscale[1] + 1e-30)) : d.N;
   float wmax = 0.f;  // running max|p| of this lane's columns (stats)
 
   for (int cbase = blockIdx.x * per_iter; cbase < ncols; cbase += gridDim.x * per_iter) {
@@ -253,13 +255,13 @@ __global__ __launch_bounds__(256) void k_column(float2* __restrict__ spec, Table
 #pragma unroll
           for (int i = 0; i < S; ++i) mx = fmaxf(mx, fabsf(u[i]));
           for (int o = LPC / 2; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
-          const double cfl = d.N * (double)mx * d.dz / d.dT;
+          const double cfl = Nn * (double)mx * d.dz / d.dT;
           if (!(cfl <= (double)d.cfl_target || mx < 1e-30f)) nsub = (int)ceil(cfl / (double)d.cfl_target);
         }
         int nsub_w = nsub;
         for (int o = 32; o >= LPC; o >>= 1) nsub_w = max(nsub_w, __shfl_xor(nsub_w, o, 64));
         const float dz = (float)(d.dz / nsub);
-        const float coef = (float)(d.N / d.dT);
+        const float coef = (float)(Nn / d.dT);
         const int lsrc = lane - 1, rsrc = lane + 1;
         const bool has_l = ln > 0, has_r = ln < LPC - 1;
         for (int sub = 0; sub < nsub_w; ++sub) {
@@ -300,7 +302,7 @@ __global__ __launch_bounds__(256) void k_column(float2* __restrict__ spec, Table
 #pragma unroll
               for (int i = 0; i < S; ++i) h[i + 1] = u[i];
             }
-            const float cf = (float)(d.N * (double)dz / d.dT);
+            const float cf = (float)(Nn * (double)dz / d.dT);
             float fl[S + 1];
 #pragma unroll
             for (int i = 0; i < S + 1; ++i) {

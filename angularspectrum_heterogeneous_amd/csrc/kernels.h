@@ -587,6 +587,7 @@ struct ColDesc {
   const float* rin;  // COL_LOAD / COL_RAWNL input [nX*nY*nT]
   float* rout;       // COL_STORE / COL_RAWNL / COL_TOREAL / COL_FINAL(optional) output
   const int32_t* cflag;  // when set, only columns with cflag[c] != 0 are processed
+  const double* nscale;  // when set, N is scaled by nscale[0] / (nscale[1] + 1e-30) (beam obliquity)
   // reductions (COL_FINAL / COL_LOAD)
   const int* row;    // device step counter; outputs go to row `*row`
   int max_rows;
@@ -670,6 +671,63 @@ __device__ __forceinline__ void kt_rhs_local(const float* u, int t0, int nT, flo
 namespace ash {
 
 __global__ void k_set_row(int* row, int v) { *row = v; }
+__global__ void k_zero2(double* a) { a[0] = 0.0; a[1] = 0.0; }
+
+// y transform of one (kx row, f block) tile and the power-weighted obliquity
+// sums of its plane-wave content (ASS:630-643): acc[0] += sum P k/k_z and
+// acc[1] += sum P over the propagating modes, P = |F|^2, k = 2 pi f / c0,
+// k_z = max(sqrt(k^2 - kx^2 - ky^2), 1e-6 k); f = 0 and evanescent modes
+// weigh 0 (ASS:598-627).  The state is only read.
+template <int L, int V>
+__global__ __launch_bounds__(LNT) void k_ypower(const float2* __restrict__ spec, Tables T, Geom g,
+                                                double* __restrict__ acc) {
+  __shared__ float2 tw[L];
+  __shared__ float2 buf[L * LF];
+  __shared__ double sred[2][LNT / 64];
+  const int tid = threadIdx.x;
+  const int x = blockIdx.x / g.nfb, fb = blockIdx.x % g.nfb;
+  const int f0 = fb * LF;
+  const float2* base = spec + (long)x * g.nY * g.MP + f0;
+  const int nvalid = min(LF, g.M + 1 - f0);
+  load_tw<L>(tw, T.twy, tid, LNT);
+  __syncthreads();
+  using LY = Lay<L, LF, 0, true>;
+  auto ld = [&](int line, int pos) -> float2 {
+    return line < nvalid ? base[(long)pos * g.MP + line] : make_float2(0.f, 0.f);
+  };
+  auto st = [&](int line, int pos, float2 v) { buf[LY::idx(line, pos)] = v; };
+  fft_run<L, false, LNT, LF, LY, V>(buf, tw, tid, ld, st);
+  __syncthreads();
+  static_assert(LNT % LF == 0, "the line of a thread must be fixed");
+  const int line = tid % LF, m = f0 + line;
+  const double km = (line < nvalid && m > 0) ? T.ka[m] : 0.0;
+  const double kx = T.kx[x];
+  double num = 0.0, den = 0.0;
+  if (km > 0.0) {
+    for (int pos = tid / LF; pos < L; pos += LNT / LF) {
+      const double ky = T.ky[pos];
+      const double kz2 = km * km - (kx * kx + ky * ky);
+      if (kz2 > 0.0) {
+        const float2 v = buf[LY::idx(line, pos)];
+        const double P = (double)v.x * v.x + (double)v.y * v.y;
+        num += P * km / fmax(sqrt(kz2), km * 1e-6);
+        den += P;
+      }
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    num += __shfl_xor(num, o, 64);
+    den += __shfl_xor(den, o, 64);
+  }
+  if ((tid & 63) == 0) { sred[0][tid >> 6] = num; sred[1][tid >> 6] = den; }
+  __syncthreads();
+  if (tid == 0) {
+    double a = 0.0, b = 0.0;
+    for (int w = 0; w < LNT / 64; ++w) { a += sred[0][w]; b += sred[1][w]; }
+    atomicAdd(acc, a);
+    atomicAdd(acc + 1, b);
+  }
+}
 __global__ void k_advance_row(int* row) { *row += 1; }
 
 }  // namespace ash

@@ -40,6 +40,7 @@ def desc_from_params(shape, params, subcycle: str = "off", cfl_target: float = 0
     d.useObliquityCorrection = int(bool(params.useObliquityCorrection))
     d.subcycle = {"off": _lib.SUBCYCLE_OFF, "column": _lib.SUBCYCLE_COLUMN}[subcycle]
     d.cflTarget = float(cfl_target)
+    d.useNonlinearityObliquity = int(bool(params.useNonlinearityObliquity))
     return d
 
 

@@ -113,8 +113,8 @@ def angular_spectrum_solve(
             raise ValueError('tof_t_ref_peak_s is required when tof_ref_trace is provided')
         raise NotImplementedError(
             'TOF extraction needs the tof_extraction module, which the reference does not ship')
-    if params.useNonlinearityObliquity:
-        raise NotImplementedError('useNonlinearityObliquity is not implemented on the GPU path')
+    if params.useNonlinearityObliquity and params.ash_learned is not None:
+        raise ValueError('the learned nonlinear step does not take the beam obliquity scale')
 
     field0 = _as_device_field(initial_field)
     nX, nY, nT = field0.shape

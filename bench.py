@@ -14,6 +14,9 @@ Also reported:
                   measured step rate, as a fraction of 8 TB/s;
   cpu_baseline  - the FP32 NumPy/SciPy restatement of the reference march timed
                   on this host's cores over a bounded sample of the same workload;
+  batch         - 32 seeded transmit configurations of uneven size (nX=nY in
+                  {128,192,256,384}, nT in {512,1024,2048}) solved as one batch
+                  with autotuned launches, against the same solves one by one;
   learned       - the same march with the learned nonlinear step (bf16 conv net
                   on the matrix cores, KT where a column's max|dp/dt| exceeds a
                   threshold): for a sweep of thresholds, rel-L2 of the full run
@@ -176,6 +179,59 @@ def learned_leg(field_np, kw, dz, K, Wm, P, plain_ms, n_pairs, epochs):
                    "fp32 accumulate; trained on the GPU by the library's own kernels"}
 
 
+def batch_leg(n, seed, prop):
+    """32 seeded transmit configurations of uneven size solved as one batch (autotuned
+    launches), against the same instances solved one by one."""
+    import numpy as np
+    import torch
+    from angularspectrum_heterogeneous_amd import workloads as W
+    from angularspectrum_heterogeneous_amd.batch import BatchSolve
+    from angularspectrum_heterogeneous_amd.solver import SolverParams, angular_spectrum_solve
+    spec = W.config4(n=n, seed=seed, propDist=prop)
+    inst = [(f, SolverParams(**kw)) for f, kw, _ in spec]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b = BatchSolve(inst, autotune=True)
+    t_setup = time.perf_counter() - t0
+    tuned = [dict(it.plan.tuned) for it in b.items]
+    t0 = time.perf_counter()
+    res = b.run()
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    b.close()
+    work = sum(int(np.prod(f.shape)) * len(r[5]) for (f, _), r in zip(inst, res))
+    errs = []
+    t_single = 0.0
+    for (f, p), r in zip(inst, res):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ref = angular_spectrum_solve(f, p, verbose=False)
+        torch.cuda.synchronize()
+        t_single += time.perf_counter() - t0
+        e = {}
+        for name, a, c in zip(("field", "pnp", "ppp", "pI", "pIloss"), r, ref):
+            a = np.asarray(a, np.float64)
+            c = np.asarray(c, np.float64)
+            e[name] = float(np.linalg.norm(a - c) / max(np.linalg.norm(c), 1e-300))
+        errs.append(e)
+        del ref
+    worst = {k: max(e[k] for e in errs) for k in errs[0]}
+    shapes = {}
+    for f, _, m in spec:
+        key = f"{m['nX']}x{m['nX']}x{m['nT']}"
+        shapes[key] = shapes.get(key, 0) + 1
+    return {"instances": n, "seed": seed, "propDist_m": prop, "shapes": shapes,
+            "voxel_z_steps": work, "batch_s": round(t_batch, 3), "setup_and_autotune_s": round(t_setup, 3),
+            "value": work / t_batch, "unit": "voxel-z-steps/s",
+            "one_by_one_s": round(t_single, 3), "one_by_one_value": work / t_single,
+            "max_rel_l2_vs_single": worst,
+            "autotune_choices": {k: sorted({str((t["x_radix_order"], t["y_radix_order"], t["column_grid"]))
+                                            for t, (_, _, m) in zip(tuned, spec)
+                                            if f"{m['nX']}x{m['nX']}x{m['nT']}" == k}) for k in shapes},
+            "note": "wall time of BatchSolve.run() (marching, stability checks and the transfer of every "
+                    "instance's outputs to the host), one graph launch per z-step for all active instances"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -185,6 +241,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned nonlinear step leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-solve leg")
+    ap.add_argument("--batch-instances", type=int, default=32)
     ap.add_argument("--learned-pairs", type=int, default=200_000)
     ap.add_argument("--learned-epochs", type=int, default=3)
     ap.add_argument("--copy-calib", action="store_true",
@@ -260,6 +318,8 @@ def main():
         "step_roofline": {"alg_bytes_per_voxel_step": 104, "frac": round(step_frac, 4)},
         "kernels": kernels,
     }
+    if not args.no_batch:
+        result["batch"] = batch_leg(args.batch_instances, 0, 0.02)
     if not args.no_learned:
         result["learned"] = learned_leg(field_np, kw, dz, K, Wm, P, ms_step, args.learned_pairs,
                                         args.learned_epochs)

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define ASH_ABI_VERSION 1
+#define ASH_ABI_VERSION 2
 
 #define ASH_OK 0
 #define ASH_E_ARG (-1)         /* invalid argument */
@@ -75,6 +75,7 @@ typedef struct ash_desc {
   int32_t useObliquityCorrection;
   int32_t subcycle;                /* ASH_SUBCYCLE_* (extension, default off) */
   double cflTarget;                /* target CFL of sub-cycling (0.5 in ASS:842) */
+  int32_t useNonlinearityObliquity; /* N scaled by the beam's power-weighted <k/k_z> (ASS:923-965) */
 } ash_desc;
 
 typedef struct ash_plan ash_plan;

@@ -600,38 +600,42 @@ def kt_flux_subcycled_columns(field, N, dZ, dT, cfl_target=0.5):
 # ---------------------------------------------------------------------------
 # Composite march steps (ASS:874-965)
 # ---------------------------------------------------------------------------
-def march_step_sequential(field, HH, abl, afilt, N, dZ, dT):
-    """ASS:874-880."""
+def march_step_sequential(field, HH, abl, afilt, N, dZ, dT, obl_map=None):
+    """ASS:874-880; with ``obl_map`` the obliquity variant ASS:923-929."""
     field = angular_spectrum_step(field, HH, abl)
-    field = rusanov_flux_standard(field, N, dZ, dT)
+    Ne = N * beam_obliquity_scalar(field, obl_map) if obl_map is not None else N
+    field = rusanov_flux_standard(field, Ne, dZ, dT)
     return attenuation_step(field, afilt)
 
 
-def march_step_split(field, HHh, ablh, afilth, N, dZ, dT, nonlinear):
-    """ASS:883-913 (the three Strang variants differ only in ``nonlinear``)."""
+def march_step_split(field, HHh, ablh, afilth, N, dZ, dT, nonlinear, obl_map=None):
+    """ASS:883-913 (the three Strang variants differ only in ``nonlinear(u, N)``);
+    with ``obl_map`` the obliquity variants ASS:932-965, whose nonlinear step
+    takes N scaled by the beam's <k/k_z> after the first half step."""
     field = attenuation_step(field, afilth)
     field = angular_spectrum_step(field, HHh, ablh)
-    field = nonlinear(field)
+    Ne = N * beam_obliquity_scalar(field, obl_map) if obl_map is not None else N
+    field = nonlinear(field, Ne)
     field = angular_spectrum_step(field, HHh, ablh)
     return attenuation_step(field, afilth)
 
 
-def march_step_split_kt(field, HHh, ablh, afilth, N, dZ, dT):
-    """ASS:905-913."""
+def march_step_split_kt(field, HHh, ablh, afilth, N, dZ, dT, obl_map=None):
+    """ASS:905-913 (ASS:956-965 with ``obl_map``)."""
     return march_step_split(field, HHh, ablh, afilth, N, dZ, dT,
-                            lambda u: kt_flux(u, N, dZ, dT))
+                            lambda u, n: kt_flux(u, n, dZ, dT), obl_map)
 
 
-def march_step_split_tvd(field, HHh, ablh, afilth, N, dZ, dT, beta_tvd):
-    """ASS:894-902."""
+def march_step_split_tvd(field, HHh, ablh, afilth, N, dZ, dT, beta_tvd, obl_map=None):
+    """ASS:894-902 (ASS:944-953 with ``obl_map``)."""
     return march_step_split(field, HHh, ablh, afilth, N, dZ, dT,
-                            lambda u: rusanov_flux_tvd(u, N, dZ, dT, beta_tvd))
+                            lambda u, n: rusanov_flux_tvd(u, n, dZ, dT, beta_tvd), obl_map)
 
 
-def march_step_split_standard(field, HHh, ablh, afilth, N, dZ, dT):
-    """ASS:883-891."""
+def march_step_split_standard(field, HHh, ablh, afilth, N, dZ, dT, obl_map=None):
+    """ASS:883-891 (ASS:932-941 with ``obl_map``)."""
     return march_step_split(field, HHh, ablh, afilth, N, dZ, dT,
-                            lambda u: rusanov_flux_standard(u, N, dZ, dT))
+                            lambda u, n: rusanov_flux_standard(u, n, dZ, dT), obl_map)
 
 
 # ---------------------------------------------------------------------------
@@ -755,25 +759,26 @@ def beta_tvd_for(p: OracleParams, dZ_step: float) -> float:
     return max(1.0, 2.0 - p.adaptiveFilterStrength * norm_step * 10)
 
 
-def march_step(field, ops, p: OracleParams, N, dZ, kt_subcycle=None):
-    """One composite step as dispatched at ASS:1454-1492 (obliquity variants excluded)."""
+def march_step(field, ops, p: OracleParams, N, dZ, kt_subcycle=None, obl_map=None):
+    """One composite step as dispatched at ASS:1454-1492; with ``obl_map`` (the
+    precalculate_obliquity_map of ASS:1374-1377) the obliquity variants."""
     dT = p.dT
     kind = nonlinear_kind(p)
     if not p.useSplitStep:
-        return march_step_sequential(field, ops['HH'], ops['abl'], ops['afilt'], N, dZ, dT)
+        return march_step_sequential(field, ops['HH'], ops['abl'], ops['afilt'], N, dZ, dT, obl_map)
     if kind == 'kt':
         if kt_subcycle == 'column':
-            nl = lambda u: kt_flux_subcycled_columns(u, N, dZ, dT)
+            nl = lambda u, n: kt_flux_subcycled_columns(u, n, dZ, dT)
         elif kt_subcycle == 'global':
-            nl = lambda u: kt_flux_adaptive(u, N, dZ, dT)
+            nl = lambda u, n: kt_flux_adaptive(u, n, dZ, dT)
         else:
-            nl = lambda u: kt_flux(u, N, dZ, dT)
+            nl = lambda u, n: kt_flux(u, n, dZ, dT)
     elif kind == 'tvd':
         b = beta_tvd_for(p, dZ)
-        nl = lambda u: rusanov_flux_tvd(u, N, dZ, dT, b)
+        nl = lambda u, n: rusanov_flux_tvd(u, n, dZ, dT, b)
     else:
-        nl = lambda u: rusanov_flux_standard(u, N, dZ, dT)
-    return march_step_split(field, ops['HH_half'], ops['abl_half'], ops['afilt_half'], N, dZ, dT, nl)
+        nl = lambda u, n: rusanov_flux_standard(u, n, dZ, dT)
+    return march_step_split(field, ops['HH_half'], ops['abl_half'], ops['afilt_half'], N, dZ, dT, nl, obl_map)
 
 
 def angular_spectrum_solve(initial_field, p: OracleParams, dtype=np.float64,
@@ -805,8 +810,8 @@ def angular_spectrum_solve(initial_field, p: OracleParams, dtype=np.float64,
         wx = make_boundary_weights(nX, boundary_width(nX, p.boundaryFactor))
         wy = make_boundary_weights(nY, boundary_width(nY, p.boundaryFactor))
     f0_bin = p.f0 / (1.0 / (nT * p.dT))
-    if p.useNonlinearityObliquity:
-        raise NotImplementedError('beam-averaged nonlinear obliquity is not restated')
+    obl_map = (precalculate_obliquity_map(nX, nY, nT, p.dX, p.dY, p.dT, c0)
+               if p.useNonlinearityObliquity else None)
 
     def alloc(ms):
         return (np.zeros((nX, nY, ms), np.float32), np.zeros((nX, nY, ms), np.float32),
@@ -844,7 +849,7 @@ def angular_spectrum_solve(initial_field, p: OracleParams, dtype=np.float64,
         step_ops = ops
         if abs(dZ_step - dZ) > 1e-15:
             step_ops = build_operators(nX, nY, nT, p, dZ_step, dtype)
-        field = march_step(field, step_ops, p, N, dZ_step, kt_subcycle)
+        field = march_step(field, step_ops, p, N, dZ_step, kt_subcycle, obl_map)
         if abl_freq is not None:
             field = freq_weighted_boundary_step(field, abl_freq)
         if p.useSuperAbsorbing:

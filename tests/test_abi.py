@@ -77,7 +77,7 @@ def test_plan_create_rejects_unsupported_sizes_before_device_use():
     d = desc_from_params((64, 64, 256), SolverParams(dX=0.0, dY=3e-4, dT=4e-8))
     assert lib.ash_plan_create(ctypes.byref(d), ctypes.byref(h)) == -1
     assert lib.ash_plan_create(None, ctypes.byref(h)) == -1
-    assert lib.ash_abi_version() == 1
+    assert lib.ash_abi_version() == 2
 
 
 def test_solver_params_match_reference_fields_and_defaults():

@@ -118,6 +118,24 @@ def test_march_step_split_kt(shape):
     assert e < 5e-6
 
 
+@pytest.mark.parametrize("shape", SIZES[:3])
+def test_march_step_nonlinearity_obliquity(shape):
+    """The composite step with N scaled by the beam obliquity (ASS:956-965)."""
+    p = _params(shape, useNonlinearityObliquity=True)
+    nX, nY, nT = shape
+    dz = 3.08e-4
+    u = _field(shape, p0=1e6)
+    N = p.beta / (2 * p.c0 ** 3 * p.rho0)
+    plan = _plan(shape, p, dz)
+    got = plan.march_step(_dev(u)).cpu().numpy()
+    ops = O.build_operators(nX, nY, nT, p, dz, np.float64)
+    obl = O.precalculate_obliquity_map(nX, nY, nT, p.dX, p.dY, p.dT, p.c0)
+    ref = O.march_step(u.astype(np.float64), ops, p, N, dz, obl_map=obl)
+    e = rel_l2(got, ref)
+    print(shape, e)
+    assert e < 5e-6
+
+
 @pytest.mark.parametrize("shape", SIZES)
 @pytest.mark.parametrize("launch", [(1, 0, 2048), (0, 1, 512), (1, 1, 4096)])
 def test_march_step_launch_configs(shape, launch):

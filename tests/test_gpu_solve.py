@@ -130,3 +130,28 @@ def test_tof_arguments_rejected():
         angular_spectrum_solve(field, SolverParams(**kw), verbose=False, tof_ref_trace=np.zeros(256))
     with pytest.raises(NotImplementedError):
         angular_spectrum_solve(field, SolverParams(**kw), verbose=False, tof_env_ratio=0.5)
+
+
+def _focused_small(p0=1e6):
+    from angularspectrum_heterogeneous_amd import workloads as W
+    field = W.focused_piston(nX=64, nY=64, nT=256, radius=8e-3, focus=15e-3, p0=p0, ncycles=3)
+    kw = W.params_kwargs(dX=3.08e-4, dY=3.08e-4, dT=4e-8, propDist=0.008, dZmin=3.08e-4)
+    return field, kw
+
+
+@pytest.mark.parametrize("variant", ["split_kt", "split_tvd", "sequential"])
+def test_nonlinearity_obliquity(variant):
+    """useNonlinearityObliquity (ASS:923-965, :1374-1377): N scaled by the beam's power-
+    weighted <k/k_z> before every nonlinear step, on a strongly focused beam."""
+    from angularspectrum_heterogeneous_amd.solver import SolverParams, angular_spectrum_solve
+    print(f"[solve] nonlinear obliquity {variant}", flush=True)
+    field, kw = _focused_small()
+    if variant == "split_tvd":
+        kw.update(fluxScheme='rusanov', useTVD=True)
+    elif variant == "sequential":
+        kw.update(useSplitStep=False, fluxScheme='rusanov')
+    kw_obl = dict(kw, useNonlinearityObliquity=True)
+    got, _, _ = _run_both(field, kw_obl)
+    plain = angular_spectrum_solve(field, SolverParams(**kw), verbose=False)
+    # the scale matters here: the two runs differ far beyond the parity bound
+    assert rel_l2(got[0], plain[0]) > 10 * TOL

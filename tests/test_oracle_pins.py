@@ -427,3 +427,33 @@ def test_torch_twin_matches_numpy_oracle():
     for i in (1, 2, 3, 4, 6):
         assert rel_l2(b[i], a[i]) < 1e-7          # the NumPy oracle stores these as float32
     np.testing.assert_array_equal(a[5], b[5])
+
+
+def _obl_closed(k, kx, ky):
+    return k / np.sqrt(k * k - kx * kx - ky * ky)
+
+
+def test_beam_obliquity_scalar_closed_forms():
+    """ASS:598-643 on monochromatic plane waves at an exact t bin: the power sits on
+    known (kx, ky) bins of the reference's centred grid, whose natural bin j carries
+    (j + 1/2) dk and bin n - j carries (1/2 - j) dk with dk = 2 pi / ((n - 1) d)."""
+    nX = nY = 16
+    nT, dX, dT, c0, m0 = 32, 3e-4, 4e-8, 1540.0, 4
+    obl = O.precalculate_obliquity_map(nX, nY, nT, dX, dX, dT, c0)
+    k = 2 * np.pi * (m0 / (nT * dT)) / c0
+    dk = 2 * np.pi / ((nX - 1) * dX)
+    t = np.arange(nT)
+    wave = np.sin(2 * np.pi * m0 * t / nT)
+    # along z (uniform in x and y): all power at natural bin (0, 0) -> |kx| = |ky| = dk/2
+    f = np.broadcast_to(wave, (nX, nY, nT)).astype(np.float64)
+    s = O.beam_obliquity_scalar(f, obl)
+    assert abs(s - _obl_closed(k, dk / 2, dk / 2)) < 2e-7  # the map is float32 (ASS:615)
+    # a real cosine of index j in x: half the power at |kx| = (j + 1/2) dk, half at (j - 1/2) dk
+    j = 2
+    x = np.arange(nX)
+    f = np.cos(2 * np.pi * j * x / nX)[:, None, None] * np.ones(nY)[None, :, None] * wave[None, None, :]
+    s = O.beam_obliquity_scalar(f, obl)
+    want = 0.5 * (_obl_closed(k, (j + 0.5) * dk, dk / 2) + _obl_closed(k, (j - 0.5) * dk, dk / 2))
+    assert abs(s - want) < 2e-7
+    # scale invariant and >= 1
+    assert abs(O.beam_obliquity_scalar(3.7e5 * f, obl) - s) < 1e-12 and s > 1.0
