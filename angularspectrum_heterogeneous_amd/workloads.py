@@ -98,8 +98,16 @@ def config4(n=32, seed=0, propDist=0.02, sizes=(128, 192, 256, 384), lengths=(51
 
     Instance k draws nX = nY from ``sizes`` and nT from ``lengths``, a focus
     depth uniform in 30-70 mm, a lateral (x) steering of the focal point
-    uniform in +-10 mm and p0 uniform in 0.2-1.5 MPa; 1 MHz, 5 cycles, piston
-    radius 0.3 nX dX, the boundaries of the 256x256x1024 case, dZ = dX.
+    uniform in +-10 mm and p0 uniform in 0.2-1.5 MPa; 1 MHz, 5 cycles, dZ = dX,
+    Wendland and frequency-weighted boundaries (boundaryFactor 0.1).  The
+    super-absorbing pass is left off: at nT = 2048 on 384^2 grids its
+    low-frequency coefficient c0/omega (ASS:225-251) makes the reference's march
+    grow without bound within ~45 steps (the FP64 restatement does the same).
+    The piston radius is
+    0.3 nX dX capped at focus / 2.5: at f-number >= 1.25 the linear focal gain
+    k a^2 / (2 F) stays below ~36, so the focal pressure stays under the
+    solver's stability limit at dZ = dX (0.2 dT / (N dZ) ~ 54 MPa; above it the
+    solve raises StabilityError, ASS:1417-1439).
     Returns [(field, kwargs, meta)] with meta the drawn values."""
     rng = np.random.default_rng(seed)
     dX, dT = 3.08e-4, 4e-8
@@ -110,11 +118,11 @@ def config4(n=32, seed=0, propDist=0.02, sizes=(128, 192, 256, 384), lengths=(51
         focus = float(rng.uniform(30e-3, 70e-3))
         steer = float(rng.uniform(-10e-3, 10e-3))
         p0 = float(rng.uniform(0.2e6, 1.5e6))
-        radius = 0.3 * nx * dX
+        radius = min(0.3 * nx * dX, focus / 2.5)
         field = focused_piston(nX=nx, nY=nx, nT=nt, dX=dX, dT=dT, p0=p0, radius=radius, focus=focus,
                                steer=(steer, 0.0))
         kw = params_kwargs(dX=dX, dY=dX, dT=dT, propDist=propDist, dZmin=dX, boundaryFactor=0.1,
-                           useFreqWeightedBoundary=True, useSuperAbsorbing=True)
+                           useFreqWeightedBoundary=True)
         out.append((field, kw, dict(nX=nx, nT=nt, focus=focus, steer=steer, p0=p0, radius=radius)))
     return out
 
