@@ -11,7 +11,7 @@
 // D[co][t] += W_k[co][ci] X[t + k - 4][ci] with v_mfma_f32_16x16x32_bf16:
 // lane l holds A[l%16][8(l/16) + j], B[8(l/16) + j][l%16] and
 // D[4(l/16) + i][l%16].  Activations are [t][32 channels] rows of bfloat16;
-// accumulation is float32.  The weight gradients are float32 VALU sums.
+// accumulation is float32, weight gradients included.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -291,43 +291,54 @@ __global__ __launch_bounds__(256) void k_net_loss(const float* __restrict__ y, c
   if (threadIdx.x == 0) atomicAdd(loss, (sred[0] + sred[1] + sred[2] + sred[3]) * (double)inv_count);
 }
 
-// Weight and bias gradients of one layer: gW[co][ci][k] += sum_t D[t][co] X[t+k-4][ci].
-// grid (nT / TT, B), 256 threads: thread -> co = tid / 8, ci = 4 (tid % 8) .. +3.
+// Weight and bias gradients of one layer on the matrix cores: for every tap k,
+// gW[co][ci][k] += sum_t D[t][co] X[t+k-4][ci] is a 32 x 32 product with the
+// sum over t as the reduction dimension.  The block's tiles are staged in LDS
+// transposed ([channel][t], bf16): A operands (8 consecutive t of one co) are
+// single 16-byte reads; B operands start at an odd offset for odd k and are
+// assembled from 2-byte reads.  grid (nT / TT, B), 256 threads.
+constexpr int WS_D = TT + 8;   // row stride of dT (272 B, a multiple of 16 B)
+constexpr int WS_X = TT + 16;  // row stride of xT (288 B)
 __global__ __launch_bounds__(256) void k_net_wgrad(const u16* __restrict__ Dg, const u16* __restrict__ X, int nT,
                                                    float* __restrict__ gW, float* __restrict__ gb) {
-  __shared__ float sD[TT][C];
-  __shared__ float sX[TT + 2 * HW][C + 1];
-  const int tid = threadIdx.x, b = blockIdx.y, t0 = blockIdx.x * TT;
+  __shared__ __align__(16) u16 dT[C * WS_D];  // dT[co][p] = D[t0 + p][co]
+  __shared__ __align__(16) u16 xT[C * WS_X];  // xT[ci][p] = X[t0 - 4 + p][ci]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int b = blockIdx.y, t0 = blockIdx.x * TT;
   const long cb = (long)b * nT;
   for (int i = tid; i < TT * C; i += 256) {
-    const int p = i / C, ch = i % C, t = t0 + p;
-    sD[p][ch] = t < nT ? bf2f(Dg[(cb + t) * C + ch]) : 0.f;
+    const int pp = i / C, ch = i % C, t = t0 + pp;
+    dT[ch * WS_D + pp] = t < nT ? Dg[(cb + t) * C + ch] : (u16)0;
   }
   for (int i = tid; i < (TT + 2 * HW) * C; i += 256) {
-    const int p = i / C, ch = i % C, t = t0 - HW + p;
-    sX[p][ch] = (t >= 0 && t < nT) ? bf2f(X[(cb + t) * C + ch]) : 0.f;
+    const int pp = i / C, ch = i % C, t = t0 - HW + pp;
+    xT[ch * WS_X + pp] = (t >= 0 && t < nT) ? X[(cb + t) * C + ch] : (u16)0;
   }
   __syncthreads();
-  const int co = tid >> 3, c0 = 4 * (tid & 7);
-  float acc[4][KW];
+  const int g = lane >> 4, r = lane & 15;
+  // 9 taps x 2 co tiles x 2 ci tiles of 16 x 16
+  for (int u = wave; u < KW * 4; u += 4) {
+    const int k = u >> 2, mt = (u >> 1) & 1, nt = u & 1;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int q = 0; q < 4; ++q)
+    for (int kb = 0; kb < TT / 32; ++kb) {
+      const int tb = 32 * kb + 8 * g;
+      const bf16x8 av = *reinterpret_cast<const bf16x8*>(dT + (16 * mt + r) * WS_D + tb);
+      const u16* xr = xT + (16 * nt + r) * WS_X + tb + k;
+      bf16x8 bv;
 #pragma unroll
-    for (int k = 0; k < KW; ++k) acc[q][k] = 0.f;
-  float accb = 0.f;
-  for (int p = 0; p < TT; ++p) {
-    const float d = sD[p][co];
-    accb += d;
+      for (int j = 0; j < 8; ++j) bv[j] = __builtin_bit_cast(__bf16, xr[j]);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
+    }
+    // D[co = 16 mt + 4 g + i][ci = 16 nt + r]
 #pragma unroll
-    for (int k = 0; k < KW; ++k)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) acc[q][k] = fmaf(d, sX[p + k][c0 + q], acc[q][k]);
+    for (int i = 0; i < 4; ++i) atomicAdd(&gW[((16 * mt + 4 * g + i) * C + 16 * nt + r) * KW + k], acc[i]);
   }
-#pragma unroll
-  for (int q = 0; q < 4; ++q)
-#pragma unroll
-    for (int k = 0; k < KW; ++k) atomicAdd(&gW[(co * C + c0 + q) * KW + k], acc[q][k]);
-  if ((tid & 7) == 0) atomicAdd(&gb[co], accb);
+  if (tid < C) {
+    float s = 0.f;
+    for (int pp = 0; pp < TT; ++pp) s += bf2f(dT[tid * WS_D + pp]);
+    atomicAdd(&gb[tid], s);
+  }
 }
 
 // Adam on the trainable entries (mask 1); masked entries stay at zero.
