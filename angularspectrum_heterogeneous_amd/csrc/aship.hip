@@ -73,11 +73,6 @@ std::vector<double> centred_natural(int n, double d, double extra_div) {
   return nat;
 }
 
-struct DevBuf {
-  void* p = nullptr;
-  size_t n = 0;
-};
-
 }  // namespace
 
 struct ash_plan {
@@ -90,7 +85,6 @@ struct ash_plan {
   float2* spec = nullptr;
   double* ibefore = nullptr;
   int* d_row = nullptr;
-  unsigned* scratch_max = nullptr;
   double* obl_acc = nullptr;  // [2] power-weighted obliquity sums of the current step
   Tables T{};
   std::vector<void*> allocs;
@@ -108,7 +102,6 @@ struct ash_plan {
   // launch configuration (ash_plan_autotune)
   int rvx = 0, rvy = 0;  // radix order of the x- and y-line transforms
   int col_grid = 2048;   // blocks of the persistent column kernels
-  bool use_graph = true;
   // per-kernel timing (ash_profile_step): events around each launch
   bool prof_on = false;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_ev;
@@ -744,7 +737,7 @@ int ash_plan_create(const ash_desc* desc, ash_plan** out) {
   hipError_t e;
   if ((e = hipMalloc(&p->spec, nspec * sizeof(float2))) != hipSuccess ||
       (e = hipMalloc(&p->ibefore, (size_t)d.nX * d.nY * sizeof(double))) != hipSuccess ||
-      (e = hipMalloc(&p->d_row, 64)) != hipSuccess || (e = hipMalloc(&p->scratch_max, 64)) != hipSuccess ||
+      (e = hipMalloc(&p->d_row, 64)) != hipSuccess ||
       (e = hipMalloc(&p->obl_acc, 64)) != hipSuccess) {
     ash_plan_destroy(p);
     return fail(ASH_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
@@ -752,7 +745,6 @@ int ash_plan_create(const ash_desc* desc, ash_plan** out) {
   p->allocs.push_back(p->spec);
   p->allocs.push_back(p->ibefore);
   p->allocs.push_back(p->d_row);
-  p->allocs.push_back(p->scratch_max);
   p->allocs.push_back(p->obl_acc);
   // padding bins are read (masked) but never written: clear once
   if ((e = hipMemset(p->spec, 0, nspec * sizeof(float2))) != hipSuccess ||
@@ -832,7 +824,7 @@ int ash_step(ash_plan* p, const ash_step_io* io, void* stream) {
   if (io->row != p->host_row) {
     hipLaunchKernelGGL(k_set_row, dim3(1), dim3(1), 0, s, p->d_row, io->row);
   }
-  if (plain_step(p, io) && p->use_graph) {
+  if (plain_step(p, io)) {
     const bool same = p->gexec && same_io(p->gkey, *io);
     if (!same) {
       if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }

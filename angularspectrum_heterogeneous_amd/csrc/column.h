@@ -118,11 +118,6 @@ __device__ __forceinline__ void cfft(float2* v, float2* cb, const float2* twdir,
 // shuffle within the wave from lane src (caller masks column edges)
 __device__ __forceinline__ float shf(float x, int src) { return __shfl(x, src, 64); }
 
-template <int S, bool MC>
-__device__ __forceinline__ void kt_rhs_regs(const float* h, int t0, int nT, float coef, float* rhs) {
-  kt_rhs_local<S, MC>(h, t0, nT, coef, rhs);
-}
-
 template <int M, int MODE, int NLK>
 __global__ __launch_bounds__(256) void k_column(float2* __restrict__ spec, Tables T, Geom g, ColDesc d) {
   using C = ColCfg<M>;

@@ -6,7 +6,7 @@
 //
 // Line sweeps (x and y) work on tiles of one line family: all positions along
 // the transform axis for LF=16 consecutive f bins (128-byte segments).  The
-// t-column kernel works on CPB whole columns per block (one wave per column).
+// t-column kernel gives each column to LPC lanes of one wave (column.h).
 #pragma once
 #include "fft_engine.h"
 
@@ -14,9 +14,6 @@ namespace ash {
 
 constexpr int LF = 16;    // f bins per line-sweep tile
 constexpr int LNT = 256;  // threads per line-sweep block
-constexpr int CPB = 4;    // columns per column-kernel block
-constexpr int TPC = 64;   // threads per column (one wave)
-constexpr int CNT = CPB * TPC;
 
 // ---------------------------------------------------------------------------
 // Generic multi-line Stockham transform.  Layout policy:
@@ -36,12 +33,9 @@ struct Lay {
   }
 };
 
-// One Stockham pass.  IN: 0 = load functor, 1 = LDS, 2 = registers rg;
-// OUT: 0 = store functor, 1 = LDS, 2 = registers rg ([q * R + r] of the
-// butterflies this thread owns).
+// One Stockham pass.  IN: 0 = load functor, 1 = LDS; OUT: 0 = store functor, 1 = LDS.
 template <int L, int P, bool INV, int NT, int NL, class LY, int IN, int OUT, int V, class LoadF, class StoreF>
-__device__ __forceinline__ void fft_pass_io(float2* buf, const float2* tw, int tid, LoadF& load, StoreF& store,
-                                            float2* rg) {
+__device__ __forceinline__ void fft_pass_io(float2* buf, const float2* tw, int tid, LoadF& load, StoreF& store) {
   using PR = PassR<L, P, INV, V>;
   constexpr int R = PR::R, NB = PR::NB, Ns = PR::Ns;
   constexpr int TOT = NB * NL;
@@ -56,14 +50,13 @@ __device__ __forceinline__ void fft_pass_io(float2* buf, const float2* tw, int t
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         if constexpr (IN == 0) v[q][r] = load(line, j + r * NB);
-        else if constexpr (IN == 1) v[q][r] = buf[LY::idx(line, j + r * NB)];
-        else v[q][r] = rg[q * R + r];
+        else v[q][r] = buf[LY::idx(line, j + r * NB)];
       }
       twiddle<L, R, Ns, INV>(v[q], j, tw);
       dft<R, INV>(v[q]);
     }
   }
-  if constexpr (OUT != 2) __syncthreads();
+  __syncthreads();
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
     const int b = tid + q * NT;
@@ -74,8 +67,7 @@ __device__ __forceinline__ void fft_pass_io(float2* buf, const float2* tw, int t
       for (int r = 0; r < R; ++r) {
         const int pos = out_pos<L, P, INV, V>(j, r);
         if constexpr (OUT == 0) store(line, pos, v[q][r]);
-        else if constexpr (OUT == 1) buf[LY::idx(line, pos)] = v[q][r];
-        else rg[q * R + r] = v[q][r];
+        else buf[LY::idx(line, pos)] = v[q][r];
       }
     }
   }
@@ -85,7 +77,7 @@ __device__ __forceinline__ void fft_pass_io(float2* buf, const float2* tw, int t
 template <int L, int P, bool INV, int NT, int NL, class LY, int V, class LoadF, class StoreF>
 __device__ __forceinline__ void fft_pass(float2* buf, const float2* tw, int tid, LoadF& load, StoreF& store) {
   constexpr int NP = PassR<L, P, INV, V>::NP;
-  fft_pass_io<L, P, INV, NT, NL, LY, P == 0 ? 0 : 1, P == NP - 1 ? 0 : 1, V>(buf, tw, tid, load, store, nullptr);
+  fft_pass_io<L, P, INV, NT, NL, LY, P == 0 ? 0 : 1, P == NP - 1 ? 0 : 1, V>(buf, tw, tid, load, store);
 }
 
 // V selects the radix order (Radix<L, V>).
