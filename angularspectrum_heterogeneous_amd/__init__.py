@@ -5,11 +5,15 @@ Drop-in for the per-z march of pinton-lab/angularspectrum_heterogeneous:
 reference's ``(field, pnp, ppp, pI, pIloss, zaxis, pax)`` outputs, running on
 hand-written HIP kernels through the C ABI of ``libaship.so``.
 """
-from .sources import SourceSlice, generate_phase_screen, make_bowl_source_planes, skull_screen
+from .sources import SourceSlice, generate_phase_screen, make_bowl_source, make_bowl_source_planes, skull_screen
+from .transcranial import (build_initial_condition, load_skull_slab, load_sparse_connector, read_nrrd,
+                           screens_to_xz_maps, skull_to_phase_screens)
 
 __all__ = [
     "SolverParams", "angular_spectrum_solve", "StabilityError", "MarchPlan",
-    "SourceSlice", "make_bowl_source_planes", "generate_phase_screen", "skull_screen",
+    "SourceSlice", "make_bowl_source", "make_bowl_source_planes", "generate_phase_screen", "skull_screen",
+    "read_nrrd", "load_skull_slab", "skull_to_phase_screens", "screens_to_xz_maps", "load_sparse_connector",
+    "build_initial_condition",
 ]
 
 

@@ -74,6 +74,7 @@ _PROTOS = {
     "ash_freq_weighted_boundary_step": ([c_vp, c_vp, c_vp, c_vp], c_i32),
     "ash_super_absorbing_step": ([c_vp, c_vp, c_vp, c_vp], c_i32),
     "ash_apply_phase_screen": ([c_vp, c_vp, c_vp, c_vp, c_vp, c_vp], c_i32),
+    "ash_source_field": ([c_vp, c_vp, c_vp, ctypes.c_int64, c_i32, c_dbl, c_dbl, c_dbl, c_dbl, c_vp, c_vp], c_i32),
     "ash_profile_step": ([c_vp, ctypes.POINTER(AshStepIO), c_i32, ctypes.POINTER(ctypes.c_float),
                           ctypes.POINTER(c_i32), c_vp], c_i32),
     "ash_plan_workspace_bytes": ([c_vp], ctypes.c_int64),

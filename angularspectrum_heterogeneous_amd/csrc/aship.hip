@@ -1117,6 +1117,18 @@ int ash_apply_phase_screen(ash_plan* p, const float* in, float* out, const float
   return check_launch();
 }
 
+int ash_source_field(const uint8_t* mask, const double* delay, const double* taxis, int64_t ncols, int32_t nT,
+                     double f0, double p0, double ncycles, double dur, float* out, void* stream) {
+  if (!mask || !delay || !taxis || !out) return fail(ASH_E_ARG, "null argument");
+  if (ncols <= 0 || nT <= 0) return fail(ASH_E_ARG, "ncols and nT must be positive");
+  if (!(ncycles != 0.0)) return fail(ASH_E_ARG, "ncycles must be nonzero");
+  const long n = (long)ncols * nT;
+  const long blocks = std::min<long>((n + 255) / 256, 16384);
+  hipLaunchKernelGGL(k_source_field, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, mask, delay, taxis,
+                     (long)ncols, (int)nT, 2.0 * M_PI * f0, ncycles, 2.0 * dur, p0, out);
+  return check_launch();
+}
+
 int ash_profile_step(ash_plan* p, const ash_step_io* io, int32_t n_iter, float* ms_out, int32_t* count_out,
                      void* stream) {
   if (!p || !io || !ms_out || !count_out || n_iter <= 0) return fail(ASH_E_ARG, "bad argument");

@@ -340,11 +340,7 @@ __global__ __launch_bounds__(256) void k_column(float2* __restrict__ spec, Table
           const double w0 = 2.0 * 3.14159265358979323846 * d.s_f0;
           const double cexp = 2.0 * d.s_dur;
 #pragma unroll
-          for (int i = 0; i < S; ++i) {
-            const double tm = d.taxis[tsamp(i)] - dl;
-            const double env = exp(-pow(1.05 * tm * w0 / (d.s_ncyc * 3.14159265358979323846), cexp));
-            u[i] += (float)(env * sin(w0 * tm) * d.s_p0);
-          }
+          for (int i = 0; i < S; ++i) u[i] += tone_burst(d.taxis[tsamp(i)] - dl, w0, d.s_ncyc, cexp, d.s_p0);
         }
       }
       float mn = u[0], mx = u[0];

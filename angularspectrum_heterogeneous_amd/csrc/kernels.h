@@ -596,6 +596,28 @@ struct ColDesc {
   double s_f0, s_p0, s_ncyc, s_dur;
 };
 
+// Tone burst of the bowl and sparse-array sources in FP64, rounded to float32 once:
+// p0 exp(-(1.05 tm w0 / (ncycles pi))^(2 dur)) sin(w0 tm), tm = t - delay
+// (ASS:1061-1064, ASS:1196-1200, validate_transcranial.py:307-309).
+__device__ __forceinline__ float tone_burst(double tm, double w0, double ncyc, double cexp, double p0) {
+  const double env = exp(-pow(1.05 * tm * w0 / (ncyc * 3.14159265358979323846), cexp));
+  return (float)(env * sin(w0 * tm) * p0);
+}
+
+// Dense field of a sparse source: out[c][t] = mask[c] ? tone_burst(taxis[t] - delay[c]) : 0.
+__global__ __launch_bounds__(256) void k_source_field(const unsigned char* __restrict__ mask,
+                                                      const double* __restrict__ delay,
+                                                      const double* __restrict__ taxis, long ncols, int nT,
+                                                      double w0, double ncyc, double cexp, double p0,
+                                                      float* __restrict__ out) {
+  const long n = ncols * (long)nT;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const long c = e / nT;
+    const int t = (int)(e - c * nT);
+    out[e] = mask[c] ? tone_burst(taxis[t] - delay[c], w0, ncyc, cexp, p0) : 0.f;
+  }
+}
+
 // MC / minmod slope of cell i from its neighbours (ASS:727-755); float32 literal ops
 __device__ __forceinline__ float sgnf(float a) { return a > 0.f ? 1.f : (a < 0.f ? -1.f : 0.f); }
 __device__ __forceinline__ float minmod2f(float a, float b) {

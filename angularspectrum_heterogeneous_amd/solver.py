@@ -14,7 +14,11 @@ reference, all deliberate:
   array;
 * TOF extraction raises ``NotImplementedError``: the reference imports it from
   a module (``tof_extraction``) that it does not ship (ASS:26-29);
-* ``useNonlinearityObliquity=True`` raises ``NotImplementedError``.
+* ``useNonlinearityObliquity=True`` together with the learned nonlinear step
+  (``ash_learned``) raises ``ValueError``;
+* an initial field or source plane given as a ``SourceSlice`` (the sparse
+  form of ``make_bowl_source``, ``make_bowl_source_planes`` and the sparse-array
+  builder) is synthesised on the GPU instead of being uploaded dense.
 """
 from __future__ import annotations
 
@@ -79,6 +83,8 @@ class StabilityError(RuntimeError):
 
 
 def _as_device_field(x) -> torch.Tensor:
+    if isinstance(x, SourceSlice):
+        return x.device_field("cuda")
     if isinstance(x, torch.Tensor):
         return x.to(device="cuda", dtype=torch.float32).contiguous()
     arr = np.ascontiguousarray(np.asarray(x), dtype=np.float32)

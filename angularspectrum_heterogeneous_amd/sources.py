@@ -1,4 +1,4 @@
-"""Input builders: bowl source planes (ASS:1080-1211) and random phase screens (ASS:290-349).
+"""Input builders: bowl sources (ASS:971-1211) and random phase screens (ASS:290-349).
 
 ``make_bowl_source_planes`` keeps the reference signature and return value
 ``(list[(z_mid, slice)], bowl_depth)``.  Each slice is a ``SourceSlice``: an
@@ -6,7 +6,10 @@ array-like object that materialises the reference's dense float32
 (nX, nY, nT) field on ``np.asarray`` (so ``planes[0][1]`` works as an initial
 field), while the solver injects it from its sparse form (a 2-D mask and delay
 map) and synthesises the tone burst on the GPU.  At 384x384x1024 a dense slice
-is 576 MiB; the sparse form is 1.3 MiB.
+is 576 MiB; the sparse form is 1.3 MiB.  ``make_bowl_source`` (the one-plane
+bowl) returns the dense field like the reference, or its ``SourceSlice`` with
+``sparse=True``; the solver synthesises a ``SourceSlice`` initial field on the
+GPU (``ash_source_field``).
 """
 from __future__ import annotations
 
@@ -51,6 +54,21 @@ class SourceSlice:
     def astype(self, dtype):
         return self.dense().astype(dtype)
 
+    def device_field(self, device="cuda"):
+        """The dense slice synthesised on the GPU (``ash_source_field``), float32 (nX, nY, nT)."""
+        import torch
+
+        from . import _lib
+        lib = _lib.load()
+        mask, delay, tax = self.device_arrays(device)
+        out = torch.empty(self.shape, dtype=torch.float32, device=device)
+        ncols = self.shape[0] * self.shape[1]
+        _lib.check(lib.ash_source_field(mask.data_ptr(), delay.data_ptr(), tax.data_ptr(), ncols, self.shape[2],
+                                        self.f0, self.p0, self.ncycles, float(self.dur), out.data_ptr(),
+                                        _lib.current_stream()), "ash_source_field")
+        torch.cuda.current_stream().synchronize()  # mask/delay/taxis are freed on return
+        return out
+
     def device_arrays(self, device):
         import torch
         mask = torch.from_numpy(np.ascontiguousarray(self.mask.astype(np.uint8))).to(device)
@@ -78,9 +96,10 @@ class SourceSlice:
         return SourceSlice(mask, delays, first.taxis, first.f0, first.p0, first.ncycles, first.dur)
 
 
-def bowl_geometry(xaxis, yaxis, c0, radius, roc, dZ, inner_radius=0.0, focus=None,
+def bowl_geometry(xaxis, yaxis, c0, radius, roc, dZ=None, inner_radius=0.0, focus=None,
                   element_delays=None, n_elements=1):
-    """Aperture, bowl depth map, inverted focusing delays and slice edges (ASS:1126-1166)."""
+    """Aperture, bowl depth map, inverted focusing delays and slice edges (ASS:1126-1166;
+    the same geometry and delays as ASS:1023-1056).  ``z_edges`` is None when ``dZ`` is."""
     if focus is None:
         focus = roc
     R = roc
@@ -104,9 +123,30 @@ def bowl_geometry(xaxis, yaxis, c0, radius, roc, dZ, inner_radius=0.0, focus=Non
                 if element_delays is not None:
                     delays_full[m] += np.asarray(element_delays)[ie]
     delays_full[aperture] = delays_full[aperture].max() - delays_full[aperture]
+    if dZ is None:
+        return aperture, bowl_z, delays_full, None, bowl_depth
     n_slices = max(1, int(np.ceil(bowl_depth / dZ)))
     z_edges = np.linspace(0, bowl_depth, n_slices + 1)
     return aperture, bowl_z, delays_full, z_edges, bowl_depth
+
+
+def make_bowl_source(xaxis, yaxis, taxis, f0, c0, p0, radius, roc, ncycles=3, dur=2, inner_radius=0.0,
+                     focus=None, element_delays=None, n_elements=1, sparse=False, verbose=False):
+    """Bowl source on one plane (ASS:971-1077): every aperture pixel fires the tone
+    burst at its inverted focusing delay (outermost first).  Returns the dense
+    float32 (nX, nY, nT) field, or with ``sparse=True`` its ``SourceSlice``
+    (aperture mask + delay map), which the solver synthesises on the GPU."""
+    aperture, _, delays, _, _ = bowl_geometry(xaxis, yaxis, c0, radius, roc, None, inner_radius, focus,
+                                              element_delays, n_elements)
+    delays = np.where(aperture, delays, 0.0)
+    sl = SourceSlice(aperture, delays, taxis, f0, p0, ncycles, dur)
+    if verbose:
+        act = delays[aperture]
+        print(f'  Bowl source: outer_r={radius*1e3:.1f} mm, inner_r={inner_radius*1e3:.1f} mm, '
+              f'ROC={roc*1e3:.1f} mm, focus={(roc if focus is None else focus)*1e3:.1f} mm, '
+              f'{n_elements} elements, {int(aperture.sum())} active grid points, '
+              f'delay spread: {(act.max() - act.min())*1e6:.2f} us')
+    return sl if sparse else sl.dense()
 
 
 def make_bowl_source_planes(xaxis, yaxis, taxis, f0, c0, p0, radius, roc, dZ, ncycles=3, dur=2,

@@ -176,6 +176,16 @@ int ash_super_absorbing_step(ash_plan* plan, const float* in_dev, float* out_dev
 int ash_apply_phase_screen(ash_plan* plan, const float* in_dev, float* out_dev, const float* phase_dev,
                            const float* amp_dev, void* stream);
 
+/* Dense initial field of a sparse source, synthesised on the device:
+ * out[c][t] = mask[c] ? p0 exp(-(1.05 (taxis[t]-delay[c]) w0 / (ncycles pi))^(2 dur))
+ *                       sin(w0 (taxis[t]-delay[c])) : 0,  w0 = 2 pi f0,
+ * in FP64 rounded once to float32 (c = x*nY + y, ncols = nX*nY, out is
+ * [ncols][nT]).  Replaces the NumPy field synthesis of make_bowl_source
+ * (ASS:1059-1064), make_bowl_source_planes (ASS:1193-1202) and the sparse-array
+ * builder (validate_transcranial.py:307-311); all pointers are device memory. */
+int ash_source_field(const uint8_t* mask_dev, const double* delay_dev, const double* taxis_dev, int64_t ncols,
+                     int32_t nT, double f0, double p0, double ncycles, double dur, float* out_dev, void* stream);
+
 /* Kernel kinds reported by ash_profile_step */
 #define ASH_K_XFWD 0        /* forward FFT along x */
 #define ASH_K_YPASS 1       /* FFT along y, operator multiply, inverse FFT */

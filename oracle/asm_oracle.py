@@ -704,6 +704,79 @@ def make_bowl_source_planes(xaxis, yaxis, taxis, f0, c0, p0, radius, roc, dZ,
     return planes, depth
 
 
+def make_bowl_source(xaxis, yaxis, taxis, f0, c0, p0, radius, roc, ncycles=3, dur=2,
+                     inner_radius=0.0, focus=None, element_delays=None, n_elements=1):
+    """ASS:971-1077: one-plane bowl source, dense float32 (nX, nY, nT)."""
+    if focus is None:
+        focus = roc
+    XX, YY = np.meshgrid(np.asarray(xaxis, dtype=np.float64), np.asarray(yaxis, dtype=np.float64), indexing='ij')
+    r2 = XX ** 2 + YY ** 2
+    r = np.sqrt(r2)
+    ap = ((r2 <= radius ** 2) & (r2 >= inner_radius ** 2)).astype(np.float32)
+    zb = roc - np.sqrt(roc ** 2 - np.minimum(r2, roc ** 2 - 1e-20))
+    dist = np.sqrt(r2 + (focus - zb) ** 2)
+    on = ap > 0
+    delays = np.zeros_like(r)
+    delays[on] = dist[on] / c0
+    if n_elements > 1:
+        dr = (radius - inner_radius) / n_elements
+        for ie in range(n_elements):
+            m = (r >= inner_radius + ie * dr) & (r < inner_radius + (ie + 1) * dr) & on
+            if m.any():
+                delays[m] = delays[m].mean()
+                if element_delays is not None:
+                    delays[m] += np.asarray(element_delays)[ie]
+    delays[on] = delays[on].max() - delays[on]
+    taxis = np.asarray(taxis, dtype=np.float64)
+    fld = ap[:, :, None] * tone_burst(taxis[None, None, :] - delays[:, :, None], f0, p0, ncycles, dur)
+    return fld.astype(np.float32)
+
+
+# ---------------------------------------------------------------------------
+# Transcranial inputs (validate_transcranial.py)
+# ---------------------------------------------------------------------------
+def skull_to_phase_screens(c_map, dz_screen, c0, f0, rho_map=None, rho0=1000.0, alpha_bone=8.0,
+                           alpha_tissue=0.5):
+    """validate_transcranial.py:166-238: [(z, phase, amplitude)] of the non-trivial slices."""
+    out = []
+    for iz in range(c_map.shape[2]):
+        c = c_map[:, :, iz]
+        phase = 2 * np.pi * f0 * dz_screen * (1.0 / c - 1.0 / c0)
+        bone = np.clip((c - 1540.0) / (2900.0 - 1540.0), 0, 1)
+        alpha = alpha_tissue + bone * (alpha_bone - alpha_tissue)
+        amp = np.exp(-(alpha * (f0 / 1e6) * 0.1151 / 0.01) * dz_screen)
+        if rho_map is not None:
+            zl = rho_map[:, :, iz] * c
+            amp = amp * np.abs(2.0 * zl / (zl + rho0 * c0))
+        if np.max(np.abs(phase)) > 1e-6 or np.min(amp) < 0.999:
+            out.append((iz * dz_screen, phase.astype(np.float32), amp.astype(np.float32)))
+    return out
+
+
+def build_initial_condition(positions, element_ids, xaxis, yaxis, t, f0, c0, p0, focal_depth, ncycles=2):
+    """validate_transcranial.py:261-317, with its per-pixel element loop."""
+    nX, nY = len(xaxis), len(yaxis)
+    dX = xaxis[1] - xaxis[0]
+    iy_raw = np.round(positions[:, 0] / dX).astype(int)
+    iz_raw = np.round(positions[:, 1] / dX).astype(int)
+    iyc = iy_raw - int(np.mean(iy_raw)) + nX // 2
+    izc = iz_raw - int(np.mean(iz_raw)) + nY // 2
+    dist = np.sqrt(focal_depth ** 2 + (iyc * dX - (nX // 2) * dX) ** 2 + (izc * dX - (nY // 2) * dX) ** 2)
+    delays = (dist.max() - dist) / c0
+    half = max(1, round(750e-6 / dX)) // 2
+    ap = np.zeros((nX, nY), dtype=np.float32)
+    tt = np.zeros((nX, nY), dtype=np.float32)
+    for m in range(len(element_ids)):
+        for di in range(-half, half + 1):
+            for dj in range(-half, half + 1):
+                yi, zi = iyc[m] + di, izc[m] + dj
+                if 0 <= yi < nX and 0 <= zi < nY:
+                    ap[yi, zi] = 1.0
+                    tt[yi, zi] = delays[m]
+    fld = tone_burst(np.asarray(t, np.float64)[None, None, :] - tt[:, :, None], f0, p0, ncycles, 2)
+    return (ap[:, :, None] * fld).astype(np.float32)
+
+
 # ---------------------------------------------------------------------------
 # Driver (ASS:1217-1567)
 # ---------------------------------------------------------------------------

@@ -77,6 +77,8 @@ def test_plan_create_rejects_unsupported_sizes_before_device_use():
     d = desc_from_params((64, 64, 256), SolverParams(dX=0.0, dY=3e-4, dT=4e-8))
     assert lib.ash_plan_create(ctypes.byref(d), ctypes.byref(h)) == -1
     assert lib.ash_plan_create(None, ctypes.byref(h)) == -1
+    assert lib.ash_source_field(None, None, None, 4, 8, 1e6, 1.0, 3.0, 2.0, None, None) == -1
+    assert b"null" in lib.ash_last_error()
     assert lib.ash_abi_version() == 2
 
 
