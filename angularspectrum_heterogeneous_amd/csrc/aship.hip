@@ -734,6 +734,11 @@ int ash_plan_create(const ash_desc* desc, ash_plan** out) {
   int rc = build_tables(p);
   if (rc) { ash_plan_destroy(p); return rc; }
   const size_t nspec = (size_t)d.nX * d.nY * p->g.MP;
+  // the line sweeps address the spectrum with 32-bit byte offsets
+  if (nspec * sizeof(float2) >= (size_t(1) << 32)) {
+    ash_plan_destroy(p);
+    return fail(ASH_E_UNSUPPORTED, "spectrum larger than 4 GiB");
+  }
   hipError_t e;
   if ((e = hipMalloc(&p->spec, nspec * sizeof(float2))) != hipSuccess ||
       (e = hipMalloc(&p->ibefore, (size_t)d.nX * d.nY * sizeof(double))) != hipSuccess ||

@@ -94,6 +94,17 @@ __device__ __forceinline__ void load_tw(float2* tw, const float2* __restrict__ g
   for (int i = tid; i < L; i += nt) tw[i] = g[i];
 }
 
+// Line-sweep element access at a 32-bit byte offset from the tile's base (a
+// block-uniform pointer): the address is one 32-bit VGPR plus the base in
+// SGPRs, instead of a 64-bit per-lane pointer.  The spectrum of the largest
+// grid (384 x 384 x 2048, 1.2 GB) stays below 4 GiB.
+__device__ __forceinline__ float2 gld(const float2* base, uint32_t off) {
+  return *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(base) + off);
+}
+__device__ __forceinline__ void gst(float2* base, uint32_t off, float2 v) {
+  *reinterpret_cast<float2*>(reinterpret_cast<char*>(base) + off) = v;
+}
+
 // ---------------------------------------------------------------------------
 // Per-plan constant tables (device memory, built on the host in FP64).
 // ---------------------------------------------------------------------------
@@ -265,17 +276,17 @@ __global__ __launch_bounds__(LNT) void k_xfft_fwd(float2* __restrict__ spec, Tab
   const int tid = threadIdx.x;
   const int y = blockIdx.x / g.nfb, fb = blockIdx.x % g.nfb;
   const int f0 = fb * LF;
-  const long sx = (long)g.nY * g.MP;
+  const uint32_t sxb = (uint32_t)g.nY * g.MP * 8u;  // x stride in bytes
   float2* base = spec + (long)y * g.MP + f0;
   const int nvalid = min(LF, g.M + 1 - f0);
   load_tw<L>(tw, T.twx, tid, LNT);
   __syncthreads();
   using LY = Lay<L, LF, 0, true>;
   auto ld = [&](int line, int pos) -> float2 {
-    return line < nvalid ? base[pos * sx + line] : make_float2(0.f, 0.f);
+    return line < nvalid ? gld(base, pos * sxb + line * 8u) : make_float2(0.f, 0.f);
   };
   auto st = [&](int line, int pos, float2 v) {
-    if (line < nvalid) base[pos * sx + line] = v;
+    if (line < nvalid) gst(base, pos * sxb + line * 8u, v);
   };
   fft_run<L, false, LNT, LF, LY, V>(buf, tw, tid, ld, st);
 }
@@ -301,7 +312,7 @@ __global__ __launch_bounds__(LNT) void k_ypass(float2* __restrict__ spec, Tables
   const int tid = threadIdx.x;
   const int ip = blockIdx.x / g.nfb, fb = blockIdx.x % g.nfb;
   const int f0 = fb * LF;
-  const long sy = g.MP;
+  const uint32_t syb = (uint32_t)g.MP * 8u;  // y stride in bytes
   const int nvalid = min(LF, g.M + 1 - f0);
   load_tw<L>(tw, T.twy, tid, LNT);
   if (tid < LF) {
@@ -346,7 +357,7 @@ __global__ __launch_bounds__(LNT) void k_ypass(float2* __restrict__ spec, Tables
     const int x = h == 0 ? ip : g.nX - 1 - ip;
     float2* base = spec + (long)x * g.nY * g.MP + f0;
     auto ld = [&](int line, int pos) -> float2 {
-      return line < nvalid ? base[pos * sy + line] : make_float2(0.f, 0.f);
+      return line < nvalid ? gld(base, pos * syb + line * 8u) : make_float2(0.f, 0.f);
     };
     auto st_lds = [&](int line, int pos, float2 v) { buf[LY::idx(line, pos)] = v; };
     // pass 0 stores to LDS only after its barrier, which follows every read of
@@ -357,7 +368,7 @@ __global__ __launch_bounds__(LNT) void k_ypass(float2* __restrict__ spec, Tables
       return cmul(buf[LY::idx(line, pos)], opt[i * LF + line]);
     };
     auto st = [&](int line, int pos, float2 v) {
-      if (line < nvalid) base[pos * sy + line] = v;
+      if (line < nvalid) gst(base, pos * syb + line * 8u, v);
     };
     __syncthreads();
     fft_run<L, true, LNT, LF, LY, V>(buf, tw, t, ld_op, st);
@@ -386,14 +397,14 @@ __global__ __launch_bounds__(LNT) void k_xfft_inv(float2* __restrict__ spec, Tab
   const int tid = threadIdx.x;
   const int y = blockIdx.x / g.nfb, fb = blockIdx.x % g.nfb;
   const int f0 = fb * LF;
-  const long sx = (long)g.nY * g.MP;
+  const uint32_t sxb = (uint32_t)g.nY * g.MP * 8u;  // x stride in bytes
   float2* base = spec + (long)y * g.MP + f0;
   const int nvalid = min(LF, g.M + 1 - f0);
   load_tw<L>(tw, T.twx, tid, LNT);
   __syncthreads();
   using LY = Lay<L, LF, 0, true>;
   auto ld = [&](int line, int pos) -> float2 {
-    return line < nvalid ? base[pos * sx + line] : make_float2(0.f, 0.f);
+    return line < nvalid ? gld(base, pos * sxb + line * 8u) : make_float2(0.f, 0.f);
   };
   if constexpr (EPI == 0 || EPI == 1) {
     float ty = 1.f;
@@ -404,7 +415,7 @@ __global__ __launch_bounds__(LNT) void k_xfft_inv(float2* __restrict__ spec, Tab
     }
     auto st = [&](int line, int pos, float2 v) {
       if constexpr (EPI == 1) v = cscale(v, tx[pos] * ty);
-      if (line < nvalid) base[pos * sx + line] = v;
+      if (line < nvalid) gst(base, pos * sxb + line * 8u, v);
     };
     fft_run<L, true, LNT, LF, LY, V>(buf, tw, tid, ld, st);
   } else {
@@ -463,7 +474,7 @@ __global__ __launch_bounds__(LNT) void k_xfft_inv(float2* __restrict__ spec, Tab
     }
     if (line < nvalid) {
 #pragma unroll
-      for (int q = 0; q < PER; ++q) base[((tid + q * LNT) / LF) * sx + line] = res[q];
+      for (int q = 0; q < PER; ++q) gst(base, ((tid + q * LNT) / LF) * sxb + line * 8u, res[q]);
     }
   }
 }
