@@ -49,6 +49,13 @@ __device__ __forceinline__ void load_a(const u16* Wl, int lane, bf16x8 (&a)[2][K
       a[mt][k] = *reinterpret_cast<const bf16x8*>(Wl + (k * C + 16 * mt + (lane & 15)) * C + 8 * (lane >> 4));
 }
 
+// A operands of the output layer, stored as [KW][16][C] (its first 16 output rows).
+__device__ __forceinline__ void load_a16(const u16* Wl, int lane, bf16x8 (&a)[2][KW]) {
+#pragma unroll
+  for (int k = 0; k < KW; ++k)
+    a[0][k] = *reinterpret_cast<const bf16x8*>(Wl + (k * 16 + (lane & 15)) * C + 8 * (lane >> 4));
+}
+
 // 32 (or 16) output channels x 16 positions p0..p0+15 from LDS rows p0-4 .. p0+19.
 template <int NMT>
 __device__ __forceinline__ void conv_tile(const u16* in, int p0, int lane, const bf16x8 (&a)[2][KW], f32x4 (&acc)[2]) {
@@ -64,25 +71,52 @@ __device__ __forceinline__ void conv_tile(const u16* in, int p0, int lane, const
 
 // ---------------------------------------------------------------------------
 // Fused inference over real columns R[c][t] (in place), persistent blocks.
-// Each column: A = max|u| and the largest slope max|u[t+1]-u[t]| / dT in one
-// pass; a column whose slope exceeds dpdt_max is flagged (flag[c] = 1) and
-// left to the KT kernel; otherwise the four layers run on chunks of TC outputs
-// with a 16-sample halo on each side, all activations in LDS.  A chunk whose
-// window stays below QUIET * A is left unchanged (the KT increment there is
-// below float32 resolution).
+// Each column: A = max|u| and the largest step D = max|u[t+1]-u[t]| in one
+// pass.  A column with D / dT > dpdt_max is flagged (flag[c] = 1) and left to
+// the KT kernel.  A column with c D < NEGLIGIBLE is left unchanged: the Burgers
+// increment of one step is bounded by about c A D (c = dz N / dT), so there it
+// is below 1e-5 of the column's own amplitude.  Otherwise the four layers run
+// on chunks of TC outputs with a 16-sample halo on each side (the receptive
+// field of four 9-tap layers), all activations in LDS; a chunk whose window
+// stays below QUIET * A is left unchanged.
 // ---------------------------------------------------------------------------
 constexpr int INF_THREADS = 512;
 constexpr int INF_WAVES = INF_THREADS / 64;
 constexpr int TC = 256;
 constexpr int ROWS = TC + 48;
 constexpr float QUIET = 1e-4f;
+constexpr float NEGLIGIBLE = 1e-5f;
+
+// First layer with its two input channels (x, x^2): the (tap, channel) pairs
+// K = 2 k + ci (18 of them, zero-padded to 32) form the reduction axis, so a
+// 16-position tile is one matrix-core product per 16 output channels instead
+// of nine.  W0p[co][K] is the packed weight, lane l's B operand holds
+// K = 8 (l/16) .. +8 of position p0 + l%16: the (x, x^2) pairs of taps
+// 4 (l/16) .. 4 (l/16) + 3 (zero past the last tap).
+__device__ __forceinline__ void conv_first_tile(const u16* in, const u16* W0p, int p0, int lane, f32x4 (&acc)[2]) {
+  const int n = lane & 15, b = lane >> 4;
+  uint32_t w[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int k = 4 * b + i;
+    if (k < KW) w[i] = *reinterpret_cast<const uint32_t*>(in + (p0 + n + k - HW) * RS);
+  }
+  const bf16x8 bv = __builtin_bit_cast(bf16x8, make_uint4(w[0], w[1], w[2], w[3]));
+#pragma unroll
+  for (int mt = 0; mt < 2; ++mt) {
+    const bf16x8 av = *reinterpret_cast<const bf16x8*>(W0p + (16 * mt + n) * C + 8 * b);
+    acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+  }
+}
 
 template <int NT_>
 __global__ __launch_bounds__(INF_THREADS) void k_net_infer(float* __restrict__ R, int ncols, const u16* __restrict__ Wm,
                                                            const float* __restrict__ params, float c, float dpdt_max,
                                                            float inv_dt, int32_t* __restrict__ flag) {
   static_assert(NT_ % TC == 0, "column length must be a multiple of the chunk");
-  __shared__ __align__(16) u16 sW[NLAY * KW * C * C];
+  // layers 1 and 2 whole, the output layer's first 16 rows, the first layer packed
+  __shared__ __align__(16) u16 sW[2 * KW * C * C + KW * 16 * C];
+  __shared__ __align__(16) u16 sW0[C * C];
   __shared__ __align__(16) u16 buf[2][ROWS * RS];
   __shared__ float scol[NT_];
   __shared__ float sbias[NLAY][C];
@@ -90,9 +124,18 @@ __global__ __launch_bounds__(INF_THREADS) void k_net_infer(float* __restrict__ R
   __shared__ float sq[INF_WAVES];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   {
-    const uint4* src = reinterpret_cast<const uint4*>(Wm);
+    // Wm is [layer][k][co][ci]; the output layer keeps co < 16 (only co = 0 is nonzero)
+    const uint4* src = reinterpret_cast<const uint4*>(Wm + W_SIZE);
     uint4* dst = reinterpret_cast<uint4*>(sW);
-    for (int i = tid; i < NLAY * KW * C * C / 8; i += INF_THREADS) dst[i] = src[i];
+    for (int i = tid; i < 2 * W_SIZE / 8; i += INF_THREADS) dst[i] = src[i];
+    for (int i = tid; i < KW * 16 * C / 8; i += INF_THREADS) {
+      const int k = i / (16 * C / 8), r = i % (16 * C / 8);
+      dst[2 * W_SIZE / 8 + i] = reinterpret_cast<const uint4*>(Wm + 3 * W_SIZE + k * C * C)[r];
+    }
+    for (int i = tid; i < C * C; i += INF_THREADS) {
+      const int co = i / C, kk = i % C, k = kk >> 1, ci = kk & 1;
+      sW0[i] = kk < 2 * KW ? Wm[(k * C + co) * C + ci] : (u16)0;
+    }
     for (int i = tid; i < NLAY * C; i += INF_THREADS) sbias[i / C][i % C] = params[(i / C) * LAYER_SIZE + W_SIZE + i % C];
   }
   __syncthreads();
@@ -116,7 +159,7 @@ __global__ __launch_bounds__(INF_THREADS) void k_net_infer(float* __restrict__ R
     for (int w = 0; w < INF_WAVES; ++w) { A = fmaxf(A, sred[0][w]); D = fmaxf(D, sred[1][w]); }
     const bool fallback = D * inv_dt > dpdt_max;  // block-uniform
     if (tid == 0) flag[col] = fallback ? 1 : 0;
-    if (fallback || A == 0.f) { __syncthreads(); continue; }
+    if (fallback || A == 0.f || c * D < NEGLIGIBLE) { __syncthreads(); continue; }
     const float gain = c * A * A;
     for (int t0 = 0; t0 < NT_; t0 += TC) {
       // features: row p is t = t0 - 16 + p
@@ -125,10 +168,8 @@ __global__ __launch_bounds__(INF_THREADS) void k_net_infer(float* __restrict__ R
         const int t = t0 - 16 + p;
         float x = 0.f;
         if (t >= 0 && t < NT_) x = scol[t] / A;
-        wm = fmaxf(wm, fabsf(x));
-        uint4* row = reinterpret_cast<uint4*>(buf[0] + p * RS);
-        row[0] = make_uint4((uint32_t)f2bf(x) | ((uint32_t)f2bf(x * x) << 16), 0u, 0u, 0u);
-        row[1] = row[2] = row[3] = make_uint4(0u, 0u, 0u, 0u);
+        if (p < TC + 32) wm = fmaxf(wm, fabsf(x));  // the window [t0 - 16, t0 + TC + 16)
+        *reinterpret_cast<uint32_t*>(buf[0] + p * RS) = (uint32_t)f2bf(x) | ((uint32_t)f2bf(x * x) << 16);
       }
       for (int o = 32; o > 0; o >>= 1) wm = fmaxf(wm, __shfl_xor(wm, o, 64));
       if (lane == 0) sq[wave] = wm;
@@ -141,34 +182,47 @@ __global__ __launch_bounds__(INF_THREADS) void k_net_infer(float* __restrict__ R
         continue;
       }
       // hidden layers j = 1..3: outputs at rows 4j .. 4j + 16 n - 1
+      auto emit = [&](u16* out, int j, int p0, const f32x4 (&acc)[2]) {
+        const int p = p0 + (lane & 15), t = t0 - 16 + p;
+        const bool inside = t >= 0 && t < NT_;
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt) {
+          const int co = 16 * mt + 4 * (lane >> 4);
+          float v[4];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) v[i] = inside ? fmaxf(acc[mt][i] + sbias[j - 1][co + i], 0.f) : 0.f;
+          *reinterpret_cast<uint2*>(out + p * RS + co) = pack4(v[0], v[1], v[2], v[3]);
+        }
+      };
+      {
+        constexpr int ntile = (TC + 32 - 8 + 15) / 16;
+        for (int tt = wave; tt < ntile; tt += INF_WAVES) {
+          const int p0 = 4 + 16 * tt;
+          f32x4 acc[2];
+          conv_first_tile(buf[0], sW0, p0, lane, acc);
+          emit(buf[1], 1, p0, acc);
+        }
+        __syncthreads();
+      }
 #pragma unroll 1
-      for (int j = 1; j <= 3; ++j) {
+      for (int j = 2; j <= 3; ++j) {
         const u16* in = buf[(j - 1) & 1];
         u16* out = buf[j & 1];
         const int first = 4 * j, ntile = (TC + 32 - 8 * j + 15) / 16;
         bf16x8 a[2][KW];
-        load_a<2>(sW + (j - 1) * KW * C * C, lane, a);
+        load_a<2>(sW + (j - 2) * KW * C * C, lane, a);
         for (int tt = wave; tt < ntile; tt += INF_WAVES) {
           const int p0 = first + 16 * tt;
           f32x4 acc[2];
           conv_tile<2>(in, p0, lane, a, acc);
-          const int p = p0 + (lane & 15), t = t0 - 16 + p;
-          const bool inside = t >= 0 && t < NT_;
-#pragma unroll
-          for (int mt = 0; mt < 2; ++mt) {
-            const int co = 16 * mt + 4 * (lane >> 4);
-            float v[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) v[i] = inside ? fmaxf(acc[mt][i] + sbias[j - 1][co + i], 0.f) : 0.f;
-            *reinterpret_cast<uint2*>(out + p * RS + co) = pack4(v[0], v[1], v[2], v[3]);
-          }
+          emit(out, j, p0, acc);
         }
         __syncthreads();
       }
       // output layer: rows 16 .. 16 + TC - 1 are t0 .. t0 + TC - 1
       {
         bf16x8 a[2][KW];
-        load_a<1>(sW + 3 * KW * C * C, lane, a);
+        load_a16(sW + 2 * KW * C * C, lane, a);
         const u16* in = buf[1];
         for (int tt = wave; tt < TC / 16; tt += INF_WAVES) {
           const int p0 = 16 + 16 * tt;

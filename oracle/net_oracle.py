@@ -174,11 +174,14 @@ def loss_and_grads(params: np.ndarray, acts, y: np.ndarray, ystar: np.ndarray, m
 
 
 def apply(params: np.ndarray, u: np.ndarray, c: float, quant: bool = True, chunk: int = 256, halo: int = 16,
-          quiet: float = 1e-4) -> np.ndarray:
+          quiet: float = 1e-4, negligible: float = 1e-5) -> np.ndarray:
     """u_new = u + c A^2 y for every column of u (B, T).
 
-    As the fused kernel: an output chunk [t0, t0 + chunk) whose window
-    [t0 - halo, t0 + chunk + halo) has max |u / A| below ``quiet`` is unchanged."""
+    As the fused kernel: a column whose largest step D = max |u[t+1] - u[t]|
+    has c D < ``negligible`` is unchanged (its Burgers increment, about c A D,
+    is below that fraction of A), and an output chunk [t0, t0 + chunk) whose
+    window [t0 - halo, t0 + chunk + halo) has max |u / A| below ``quiet`` is
+    unchanged."""
     u = np.asarray(u, dtype=np.float32)
     X0, A = features(u, quant)
     _, y = forward(params, X0, quant)
@@ -189,6 +192,8 @@ def apply(params: np.ndarray, u: np.ndarray, c: float, quant: bool = True, chunk
     for t0 in range(0, T, chunk):
         lo, hi = max(t0 - halo, 0), min(t0 + chunk + halo, T)
         keep[:, t0:t0 + chunk] = (np.max(x[:, lo:hi], axis=1) >= quiet)[:, None]
+    D = np.max(np.abs(np.diff(u, axis=1)), axis=1)
+    keep &= (np.float32(c) * D >= np.float32(negligible))[:, None]
     inc = c * (A.astype(np.float64) ** 2)[:, None] * y * keep
     return (u.astype(np.float64) + inc).astype(np.float32)
 

@@ -70,8 +70,12 @@ def test_fused_apply_matches_oracle(nT):
     u = _columns(40, nT, 5)
     # make a few columns steep (square-ish waves) so that they fall back to KT
     u[:5] = (np.sign(u[:5]) * np.abs(u[:5]) ** 0.3 * np.max(np.abs(u[:5]), axis=1, keepdims=True) ** 0.7)
+    # and a few weak enough that one step changes them by less than 1e-5 of their amplitude
+    u[5:9] *= 1e-3
     u = u.astype(np.float32)
     c = 3.7e-9
+    weak = np.float32(c) * np.max(np.abs(np.diff(u, axis=1)), axis=1) < np.float32(1e-5)
+    assert weak[5:9].all() and not weak[9:].any()
     # threshold in the middle of a wide gap between the columns' max|du/dt|
     d = np.sort(NO.max_dpdt(u, 4e-8))
     k = max(range(len(d) // 2, len(d) - 1), key=lambda i: d[i + 1] / d[i])
@@ -85,6 +89,7 @@ def test_fused_apply_matches_oracle(nT):
     ref = NO.apply(p, u, c)
     keep = ~want_flag
     assert np.array_equal(got[want_flag], u[want_flag])
+    assert np.array_equal(got[weak], u[weak]) and np.array_equal(ref[weak], u[weak])
     assert rel_l2(got[keep] - u[keep], ref[keep] - u[keep]) <= 5e-3
 
 

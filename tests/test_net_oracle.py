@@ -91,3 +91,17 @@ def test_apply_leaves_quiet_chunks_unchanged():
     # chunk 0 (outputs [0, 256), window [0, 272)) sees only zeros: unchanged
     assert np.array_equal(out[:, :256], u[:, :256])
     assert not np.array_equal(out[:, 256:512], u[:, 256:512])
+
+
+def test_apply_leaves_negligible_columns_unchanged():
+    # c max|u[t+1] - u[t]| < 1e-5: the column is left as it is, network or not
+    rng = np.random.default_rng(5)
+    p = NO.init_params(2) + (rng.normal(0, 0.05, NO.PARAM_COUNT) * NO.trainable_mask()).astype(np.float32)
+    p[3 * NO.LAYER_SIZE + NO.W_SIZE] = 0.3
+    t = np.arange(256)
+    u = np.stack([np.sin(0.2 * t), np.sin(0.2 * t)]).astype(np.float32)   # max step ~0.2
+    c = 4e-5                                                              # c D ~ 8e-6 < 1e-5
+    out = NO.apply(p, u, c)
+    assert np.array_equal(out, u)
+    out2 = NO.apply(p, 2 * u, c)                                          # c D ~ 1.6e-5
+    assert not np.array_equal(out2, 2 * u)
