@@ -82,8 +82,9 @@ __device__ __forceinline__ void conv_tile(const u16* in, int p0, int lane, const
 // ---------------------------------------------------------------------------
 constexpr int INF_THREADS = 512;
 constexpr int INF_WAVES = INF_THREADS / 64;
-constexpr int TC = 256;
-constexpr int ROWS = TC + 48;
+// chunk of outputs per pass: 512 (34/33/33/32 tiles per layer keep the 8 waves
+// busy between the layer barriers), the whole column when it is shorter
+template <int NT_> constexpr int chunk_len() { return NT_ < 512 ? NT_ : 512; }
 constexpr float QUIET = 1e-4f;
 constexpr float NEGLIGIBLE = 1e-5f;
 
@@ -113,6 +114,7 @@ template <int NT_>
 __global__ __launch_bounds__(INF_THREADS) void k_net_infer(float* __restrict__ R, int ncols, const u16* __restrict__ Wm,
                                                            const float* __restrict__ params, float c, float dpdt_max,
                                                            float inv_dt, int32_t* __restrict__ flag) {
+  constexpr int TC = chunk_len<NT_>(), ROWS = TC + 48;
   static_assert(NT_ % TC == 0, "column length must be a multiple of the chunk");
   // layers 1 and 2 whole, the output layer's first 16 rows, the first layer packed
   __shared__ __align__(16) u16 sW[2 * KW * C * C + KW * 16 * C];

@@ -279,8 +279,9 @@ int ash_net_adam(ash_net* net, const float* grads_dev, float lr, float beta1, fl
  * columns whose max|u[t+1]-u[t]| / dT is at most dpdt_max; flag[c] = 1 marks
  * the others (left unchanged).  A column with c max|u[t+1]-u[t]| < 1e-5 (a
  * Burgers increment below 1e-5 of its amplitude) is left unchanged with
- * flag 0.  Within a column, a 256-sample chunk whose window (16 samples of
- * halo each side) stays below 1e-4 A is left unchanged. */
+ * flag 0.  Within a column, a 512-sample chunk (the whole column when
+ * shorter) whose window (16 samples of halo each side) stays below 1e-4 A is
+ * left unchanged. */
 int ash_net_apply(ash_net* net, float* u_dev, int32_t ncols, double c, double dpdt_max, double dT,
                   int32_t* flag_dev, void* stream);
 

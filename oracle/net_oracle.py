@@ -173,7 +173,7 @@ def loss_and_grads(params: np.ndarray, acts, y: np.ndarray, ystar: np.ndarray, m
     return loss, grad
 
 
-def apply(params: np.ndarray, u: np.ndarray, c: float, quant: bool = True, chunk: int = 256, halo: int = 16,
+def apply(params: np.ndarray, u: np.ndarray, c: float, quant: bool = True, chunk: int = 0, halo: int = 16,
           quiet: float = 1e-4, negligible: float = 1e-5) -> np.ndarray:
     """u_new = u + c A^2 y for every column of u (B, T).
 
@@ -181,8 +181,11 @@ def apply(params: np.ndarray, u: np.ndarray, c: float, quant: bool = True, chunk
     has c D < ``negligible`` is unchanged (its Burgers increment, about c A D,
     is below that fraction of A), and an output chunk [t0, t0 + chunk) whose
     window [t0 - halo, t0 + chunk + halo) has max |u / A| below ``quiet`` is
-    unchanged."""
+    unchanged.  The kernel's chunk is 512 samples, the whole column when shorter
+    (``chunk=0``)."""
     u = np.asarray(u, dtype=np.float32)
+    if chunk <= 0:
+        chunk = min(u.shape[1], 512)
     X0, A = features(u, quant)
     _, y = forward(params, X0, quant)
     As = np.where(A > 0, A, np.float32(1.0)).astype(np.float32)

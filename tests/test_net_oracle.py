@@ -85,12 +85,12 @@ def test_apply_leaves_quiet_chunks_unchanged():
     rng = np.random.default_rng(4)
     p = NO.init_params(1) + (rng.normal(0, 0.05, NO.PARAM_COUNT) * NO.trainable_mask()).astype(np.float32)
     p[3 * NO.LAYER_SIZE + NO.W_SIZE] = 0.3  # output bias: a nonzero y everywhere
-    u = np.zeros((2, 512), np.float32)
-    u[:, 300:340] = rng.normal(size=(2, 40)).astype(np.float32)
+    u = np.zeros((2, 1024), np.float32)
+    u[:, 560:600] = rng.normal(size=(2, 40)).astype(np.float32)
     out = NO.apply(p, u, 1e-3)
-    # chunk 0 (outputs [0, 256), window [0, 272)) sees only zeros: unchanged
-    assert np.array_equal(out[:, :256], u[:, :256])
-    assert not np.array_equal(out[:, 256:512], u[:, 256:512])
+    # chunk 0 (outputs [0, 512), window [0, 528)) sees only zeros: unchanged
+    assert np.array_equal(out[:, :512], u[:, :512])
+    assert not np.array_equal(out[:, 512:], u[:, 512:])
 
 
 def test_apply_leaves_negligible_columns_unchanged():
