@@ -141,14 +141,32 @@ __global__ __launch_bounds__(INF_THREADS) void k_net_infer(float* __restrict__ R
     for (int i = tid; i < NLAY * C; i += INF_THREADS) sbias[i / C][i % C] = params[(i / C) * LAYER_SIZE + W_SIZE + i % C];
   }
   __syncthreads();
+  // the next column is read into registers while this one is computed
+  constexpr int NQ = (NT_ + INF_THREADS - 1) / INF_THREADS;
+  float nx[NQ];
+  auto fetch = [&](int cl) {
+    const float* gg = R + (long)cl * NT_;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int i = tid + q * INF_THREADS;
+      nx[q] = i < NT_ ? gg[i] : 0.f;
+    }
+  };
+  if (blockIdx.x < ncols) fetch(blockIdx.x);
   for (int col = blockIdx.x; col < ncols; col += gridDim.x) {
     float* g = R + (long)col * NT_;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int i = tid + q * INF_THREADS;
+      if (i < NT_) scol[i] = nx[q];
+    }
+    __syncthreads();
+    if (col + (int)gridDim.x < ncols) fetch(col + gridDim.x);
     float am = 0.f, dm = 0.f;
     for (int i = tid; i < NT_; i += INF_THREADS) {
-      const float v = g[i];
-      scol[i] = v;
+      const float v = scol[i];
       am = fmaxf(am, fabsf(v));
-      if (i + 1 < NT_) dm = fmaxf(dm, fabsf(g[i + 1] - v));
+      if (i + 1 < NT_) dm = fmaxf(dm, fabsf(scol[i + 1] - v));
     }
     for (int o = 32; o > 0; o >>= 1) {
       am = fmaxf(am, __shfl_xor(am, o, 64));
