@@ -49,13 +49,6 @@ __device__ __forceinline__ void load_a(const u16* Wl, int lane, bf16x8 (&a)[2][K
       a[mt][k] = *reinterpret_cast<const bf16x8*>(Wl + (k * C + 16 * mt + (lane & 15)) * C + 8 * (lane >> 4));
 }
 
-// A operands of the output layer, stored as [KW][16][C] (its first 16 output rows).
-__device__ __forceinline__ void load_a16(const u16* Wl, int lane, bf16x8 (&a)[2][KW]) {
-#pragma unroll
-  for (int k = 0; k < KW; ++k)
-    a[0][k] = *reinterpret_cast<const bf16x8*>(Wl + (k * 16 + (lane & 15)) * C + 8 * (lane >> 4));
-}
-
 // 32 (or 16) output channels x 16 positions p0..p0+15 from LDS rows p0-4 .. p0+19.
 template <int NMT>
 __device__ __forceinline__ void conv_tile(const u16* in, int p0, int lane, const bf16x8 (&a)[2][KW], f32x4 (&acc)[2]) {
@@ -116,9 +109,10 @@ __global__ __launch_bounds__(INF_THREADS) void k_net_infer(float* __restrict__ R
                                                            float inv_dt, int32_t* __restrict__ flag) {
   constexpr int TC = chunk_len<NT_>(), ROWS = TC + 48;
   static_assert(NT_ % TC == 0, "column length must be a multiple of the chunk");
-  // layers 1 and 2 whole, the output layer's first 16 rows, the first layer packed
-  __shared__ __align__(16) u16 sW[2 * KW * C * C + KW * 16 * C];
+  // layers 1 and 2 whole, the first and the output layer packed
+  __shared__ __align__(16) u16 sW[2 * KW * C * C];
   __shared__ __align__(16) u16 sW0[C * C];
+  __shared__ __align__(16) u16 sW3[16 * C];
   __shared__ __align__(16) u16 buf[2][ROWS * RS];
   __shared__ float scol[NT_];
   __shared__ float sbias[NLAY][C];
@@ -126,17 +120,18 @@ __global__ __launch_bounds__(INF_THREADS) void k_net_infer(float* __restrict__ R
   __shared__ float sq[INF_WAVES];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   {
-    // Wm is [layer][k][co][ci]; the output layer keeps co < 16 (only co = 0 is nonzero)
+    // Wm is [layer][k][co][ci]
     const uint4* src = reinterpret_cast<const uint4*>(Wm + W_SIZE);
     uint4* dst = reinterpret_cast<uint4*>(sW);
     for (int i = tid; i < 2 * W_SIZE / 8; i += INF_THREADS) dst[i] = src[i];
-    for (int i = tid; i < KW * 16 * C / 8; i += INF_THREADS) {
-      const int k = i / (16 * C / 8), r = i % (16 * C / 8);
-      dst[2 * W_SIZE / 8 + i] = reinterpret_cast<const uint4*>(Wm + 3 * W_SIZE + k * C * C)[r];
-    }
     for (int i = tid; i < C * C; i += INF_THREADS) {
       const int co = i / C, kk = i % C, k = kk >> 1, ci = kk & 1;
       sW0[i] = kk < 2 * KW ? Wm[(k * C + co) * C + ci] : (u16)0;
+    }
+    // output layer (its only output channel is 0) with the taps as rows: sW3[k][ci]
+    for (int i = tid; i < 16 * C; i += INF_THREADS) {
+      const int k = i / C, ci = i % C;
+      sW3[i] = k < KW ? Wm[3 * W_SIZE + (k * C + 0) * C + ci] : (u16)0;
     }
     for (int i = tid; i < NLAY * C; i += INF_THREADS) sbias[i / C][i % C] = params[(i / C) * LAYER_SIZE + W_SIZE + i % C];
   }
@@ -239,19 +234,40 @@ __global__ __launch_bounds__(INF_THREADS) void k_net_infer(float* __restrict__ R
         }
         __syncthreads();
       }
-      // output layer: rows 16 .. 16 + TC - 1 are t0 .. t0 + TC - 1
+      // output layer: rows 16 .. 16 + TC - 1 are t0 .. t0 + TC - 1.  With the
+      // taps as matrix rows, D[k][n] = sum_ci W[0][ci][k] h[p0 - 4 + n][ci] for
+      // 32 consecutive rows n (two products), and output m of the tile is the
+      // diagonal sum over k of D[k][m + k], gathered through LDS (buf[0], free
+      // once the last hidden layer has been read).
       {
-        bf16x8 a[2][KW];
-        load_a16(sW + 2 * KW * C * C, lane, a);
+        const bf16x8 a3 = *reinterpret_cast<const bf16x8*>(sW3 + (lane & 15) * C + 8 * (lane >> 4));
         const u16* in = buf[1];
+        float* dsc = reinterpret_cast<float*>(buf[0]) + wave * 512;
+        const int n = lane & 15, b = lane >> 4;
         for (int tt = wave; tt < TC / 16; tt += INF_WAVES) {
           const int p0 = 16 + 16 * tt;
-          f32x4 acc[2];
-          conv_tile<1>(in, p0, lane, a, acc);
-          if ((lane >> 4) == 0) {
-            const int t = t0 + 16 * tt + lane;
-            g[t] = scol[t] + gain * (acc[0][0] + sbias[3][0]);
+          const bf16x8 b1 = *reinterpret_cast<const bf16x8*>(in + (p0 + n - HW) * RS + 8 * b);
+          const bf16x8 b2 = *reinterpret_cast<const bf16x8*>(in + (p0 + n + 16 - HW) * RS + 8 * b);
+          const f32x4 z = {0.f, 0.f, 0.f, 0.f};
+          const f32x4 d1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3, b1, z, 0, 0, 0);
+          const f32x4 d2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3, b2, z, 0, 0, 0);
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            dsc[(4 * b + i) * 16 + n] = d1[i];
+            dsc[256 + (4 * b + i) * 16 + n] = d2[i];
           }
+          wave_sync();
+          if (b == 0) {
+            float sum = 0.f;
+#pragma unroll
+            for (int k = 0; k < KW; ++k) {
+              const int q = n + k;
+              sum += q < 16 ? dsc[k * 16 + q] : dsc[256 + k * 16 + q - 16];
+            }
+            const int t = t0 + 16 * tt + n;
+            g[t] = scol[t] + gain * (sum + sbias[3][0]);
+          }
+          wave_sync();
         }
       }
       __syncthreads();
