@@ -226,10 +226,22 @@ __global__ __launch_bounds__(INF_THREADS) void k_net_infer(float* __restrict__ R
         const int first = 4 * j, ntile = (TC + 32 - 8 * j + 15) / 16;
         bf16x8 a[2][KW];
         load_a<2>(sW + (j - 2) * KW * C * C, lane, a);
+        // the nine B fragments of a tile are read before its products, one wait per tile
+        const u16* rowp = in + ((lane & 15) - HW) * RS + 8 * (lane >> 4);
         for (int tt = wave; tt < ntile; tt += INF_WAVES) {
           const int p0 = first + 16 * tt;
-          f32x4 acc[2];
-          conv_tile<2>(in, p0, lane, a, acc);
+          bf16x8 cur[KW];
+#pragma unroll
+          for (int k = 0; k < KW; ++k) cur[k] = *reinterpret_cast<const bf16x8*>(rowp + (p0 + k) * RS);
+          f32x4 acc[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+          for (int k = 0; k < KW; ++k) {
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt) acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt][k], cur[k], acc[mt], 0, 0, 0);
+          }
+          // scheduling groups: the nine LDS reads, then the eighteen products
+          __builtin_amdgcn_sched_group_barrier(0x100, KW, 0);
+          __builtin_amdgcn_sched_group_barrier(0x008, 2 * KW, 0);
           emit(out, j, p0, acc);
         }
         __syncthreads();
