@@ -394,7 +394,7 @@ void launch_xfwd_L(ash_plan* p, hipStream_t s) {
 }
 template <int L>
 void launch_ypass_L(ash_plan* p, const OpDesc& op, hipStream_t s) {
-  const dim3 grid((p->g.nX / 2) * p->g.nfb);  // one block per (x row pair, f group)
+  const dim3 grid((p->g.nX / 2) * (p->g.MP / ypass_lines<L>()));  // one block per (x row pair, f group)
   // the FP64-operator reference runs with the default radix order
   if (op.precise) hipLaunchKernelGGL((k_ypass<L, true, 0>), grid, dim3(LNT), 0, s, p->spec, p->T, p->g, op);
   else if (p->rvy) hipLaunchKernelGGL((k_ypass<L, false, 1>), grid, dim3(LNT), 0, s, p->spec, p->T, p->g, op);

@@ -291,6 +291,10 @@ __global__ __launch_bounds__(LNT) void k_xfft_fwd(float2* __restrict__ spec, Tab
   fft_run<L, false, LNT, LF, LY, V>(buf, tw, tid, ld, st);
 }
 
+// f bins per y-pass tile: 8 for the 384-point line, whose 16-bin tile (78 KB of
+// LDS with the operator table) would hold the CU to two blocks
+template <int L> constexpr int ypass_lines() { return L == 384 ? 8 : LF; }
+
 // ---------------------------------------------------------------------------
 // y-pass: forward FFT along y, multiply by the visit operator, inverse FFT.
 // Tile = (kx, f-block).  The 1/(nX nY) normalisation is in the operator.
@@ -305,17 +309,19 @@ __global__ __launch_bounds__(LNT) void k_xfft_fwd(float2* __restrict__ spec, Tab
 template <int L, bool PRECISE, int V>
 __global__ __launch_bounds__(LNT) void k_ypass(float2* __restrict__ spec, Tables T, Geom g, OpDesc op) {
   constexpr int H = L / 2;
+  constexpr int LFY = ypass_lines<L>();
   __shared__ float2 tw[L];
-  __shared__ float2 buf[L * LF];
-  __shared__ float2 opt[H * LF];  // [i * LF + line]: operator at natural y bins i and L-1-i
-  __shared__ LineConst slc[LF];
+  __shared__ float2 buf[L * LFY];
+  __shared__ float2 opt[H * LFY];  // [i * LFY + line]: operator at natural y bins i and L-1-i
+  __shared__ LineConst slc[LFY];
   const int tid = threadIdx.x;
-  const int ip = blockIdx.x / g.nfb, fb = blockIdx.x % g.nfb;
-  const int f0 = fb * LF;
+  const int nfb = g.MP / LFY;
+  const int ip = blockIdx.x / nfb, fb = blockIdx.x % nfb;
+  const int f0 = fb * LFY;
   const uint32_t syb = (uint32_t)g.MP * 8u;  // y stride in bytes
-  const int nvalid = min(LF, g.M + 1 - f0);
+  const int nvalid = min(LFY, g.M + 1 - f0);
   load_tw<L>(tw, T.twy, tid, LNT);
-  if (tid < LF) {
+  if (tid < LFY) {
     const int m = min(f0 + tid, g.MP - 1);
     const double kt = T.kt[m], ka = T.ka[m];
     LineConst lc;
@@ -330,17 +336,17 @@ __global__ __launch_bounds__(LNT) void k_ypass(float2* __restrict__ spec, Tables
     slc[tid] = lc;
   }
   __syncthreads();
-  static_assert(LNT % LF == 0, "the line of a thread must be fixed");
-  const int myline = tid % LF;
+  static_assert(LNT % LFY == 0, "the line of a thread must be fixed");
+  const int myline = tid % LFY;
   const LineConst lc = slc[myline];
-  using LY = Lay<L, LF, 0, true>;
+  using LY = Lay<L, LFY, 0, true>;
   {
     const double kx = T.kx[ip];
     const double kx2 = kx * kx;
     // entry e = tid + LNT k is on line e % LF = myline
 #pragma unroll 1
-    for (int e = tid; e < H * LF; e += LNT) {
-      const double ky = T.ky[e / LF];
+    for (int e = tid; e < H * LFY; e += LNT) {
+      const double ky = T.ky[e / LFY];
       const double kk = kx2 + ky * ky;
       if constexpr (PRECISE) opt[e] = visit_operator_fp64(op, kk, lc, f0 + myline);
       else opt[e] = visit_operator(op, kk, lc, f0 + myline);
@@ -362,16 +368,16 @@ __global__ __launch_bounds__(LNT) void k_ypass(float2* __restrict__ spec, Tables
     auto st_lds = [&](int line, int pos, float2 v) { buf[LY::idx(line, pos)] = v; };
     // pass 0 stores to LDS only after its barrier, which follows every read of
     // the previous row's last pass
-    fft_run<L, false, LNT, LF, LY, V>(buf, tw, t, ld, st_lds);
+    fft_run<L, false, LNT, LFY, LY, V>(buf, tw, t, ld, st_lds);
     auto ld_op = [&](int line, int pos) -> float2 {
       const int i = pos < H ? pos : L - 1 - pos;
-      return cmul(buf[LY::idx(line, pos)], opt[i * LF + line]);
+      return cmul(buf[LY::idx(line, pos)], opt[i * LFY + line]);
     };
     auto st = [&](int line, int pos, float2 v) {
       if (line < nvalid) gst(base, pos * syb + line * 8u, v);
     };
     __syncthreads();
-    fft_run<L, true, LNT, LF, LY, V>(buf, tw, t, ld_op, st);
+    fft_run<L, true, LNT, LFY, LY, V>(buf, tw, t, ld_op, st);
   }
 }
 

@@ -13,7 +13,7 @@ from tests.conftest import rel_l2
 
 pytestmark = pytest.mark.gpu
 
-SIZES = [(64, 64, 256), (128, 192, 512), (256, 256, 1024), (384, 64, 2048)]
+SIZES = [(64, 64, 256), (128, 192, 512), (256, 256, 1024), (384, 64, 2048), (64, 384, 512)]
 
 
 def _params(shape, **over):
