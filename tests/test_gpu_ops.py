@@ -155,7 +155,7 @@ def test_march_step_launch_configs(shape, launch):
     assert e < 5e-6
 
 
-@pytest.mark.parametrize("shape", SIZES[:2])
+@pytest.mark.parametrize("shape", SIZES[:2] + SIZES[3:4])
 def test_freq_weighted_boundary(shape):
     p = _params(shape, useFreqWeightedBoundary=True)
     nX, nY, nT = shape
@@ -169,7 +169,7 @@ def test_freq_weighted_boundary(shape):
     assert e < 2e-6
 
 
-@pytest.mark.parametrize("shape", SIZES[:2])
+@pytest.mark.parametrize("shape", SIZES[:2] + SIZES[3:4])
 def test_super_absorbing(shape):
     p = _params(shape, useSuperAbsorbing=True)
     nX, nY, nT = shape
