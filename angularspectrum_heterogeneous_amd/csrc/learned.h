@@ -73,7 +73,7 @@ __device__ __forceinline__ void conv_tile(const u16* in, int p0, int lane, const
 // field of four 9-tap layers), all activations in LDS; a chunk whose window
 // stays below QUIET * A is left unchanged.
 // ---------------------------------------------------------------------------
-constexpr int INF_THREADS = 512;
+constexpr int INF_THREADS = 768;
 constexpr int INF_WAVES = INF_THREADS / 64;
 // chunk of outputs per pass: 512 (34/33/33/32 tiles per layer keep the 8 waves
 // busy between the layer barriers), the whole column when it is shorter
@@ -254,7 +254,9 @@ __global__ __launch_bounds__(INF_THREADS) void k_net_infer(float* __restrict__ R
       {
         const bf16x8 a3 = *reinterpret_cast<const bf16x8*>(sW3 + (lane & 15) * C + 8 * (lane >> 4));
         const u16* in = buf[1];
-        float* dsc = reinterpret_cast<float*>(buf[0]) + wave * 512;
+        // per wave: D1 (16 x 16) and the 8 columns of D2 that reach the tile (16 x 8)
+        static_assert(INF_WAVES * 384 * 4 <= ROWS * RS * 2, "diagonal scratch fits in one activation buffer");
+        float* dsc = reinterpret_cast<float*>(buf[0]) + wave * 384;
         const int n = lane & 15, b = lane >> 4;
         for (int tt = wave; tt < TC / 16; tt += INF_WAVES) {
           const int p0 = 16 + 16 * tt;
@@ -266,7 +268,7 @@ __global__ __launch_bounds__(INF_THREADS) void k_net_infer(float* __restrict__ R
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
             dsc[(4 * b + i) * 16 + n] = d1[i];
-            dsc[256 + (4 * b + i) * 16 + n] = d2[i];
+            if (n < 8) dsc[256 + (4 * b + i) * 8 + n] = d2[i];
           }
           wave_sync();
           if (b == 0) {
@@ -274,7 +276,7 @@ __global__ __launch_bounds__(INF_THREADS) void k_net_infer(float* __restrict__ R
 #pragma unroll
             for (int k = 0; k < KW; ++k) {
               const int q = n + k;
-              sum += q < 16 ? dsc[k * 16 + q] : dsc[256 + k * 16 + q - 16];
+              sum += q < 16 ? dsc[k * 16 + q] : dsc[256 + k * 8 + q - 16];
             }
             const int t = t0 + 16 * tt + n;
             g[t] = scol[t] + gain * (sum + sbias[3][0]);
