@@ -227,15 +227,18 @@ __device__ __forceinline__ float2 visit_operator(const OpDesc& op, double kk, co
   if (op.use_prop) {
     if (!lc.kept || kk > op.k_max2) return make_float2(0.f, 0.f);
     if (kk > op.k_start2) {
+      // pos in [0, 1] (kk <= k_max2 here): the bounded sincos, not the general cospi
       const float pos = (__builtin_amdgcn_sqrtf(kkf) - op.k_startf) * op.inv_band;
-      amp *= 0.5f * (1.f + cospif(pos));
+      float sn, cs;
+      bounded_sincos(3.14159265358979f * pos, &sn, &cs);
+      amp *= 0.5f * (1.f + cs);
     }
     const float d2 = (float)(lc.kt2 - kk);
     if (d2 > 0.f) {
       ph = op.dzp * kkf * __builtin_amdgcn_rcpf(lc.kt + __builtin_amdgcn_sqrtf(d2));
     } else {
       ph = op.dzp * lc.kt;
-      lg = -op.dzp * sqrtf(-d2);
+      lg = -op.dzp * __builtin_amdgcn_sqrtf(-d2);
     }
   }
   if (op.use_att && m > 0) {
@@ -255,12 +258,17 @@ __device__ __forceinline__ float2 visit_operator(const OpDesc& op, double kk, co
       im = -lc.as;
     }
     if (apply) {
-      if (op.clip_negative && cosf(im) < 0.f) return make_float2(0.f, 0.f);
+      if (op.clip_negative) {
+        float sn, cs;
+        bounded_sincos(im, &sn, &cs);
+        if (cs < 0.f) return make_float2(0.f, 0.f);
+      }
       lg += re;
       ph += im;
     }
   }
-  if (lg != 0.f) amp *= expf(lg);
+  // lg <= 0: the hardware base-2 exponential (v_exp_f32), ~1 ulp
+  if (lg != 0.f) amp *= __builtin_amdgcn_exp2f(lg * 1.44269504088896341f);
   float s, c;
   bounded_sincos(ph, &s, &c);
   return make_float2(amp * c, amp * s);
