@@ -67,7 +67,7 @@ def cpu_baseline(steps: int):
             "step_s": [round(s, 3) for s in tim["step_s"]]}
 
 
-def timed_march(field_np, params, dz, K, Wm, P, net=None, dpdt_max=0.0):
+def timed_march(field_np, params, dz, K, Wm, P, net=None, dpdt_max=0.0, tune=None):
     """W warm-up steps, K timed steps (synchronised on both sides), then P steps with
     HIP events around every launch.  Returns (seconds, per-kernel dict, max |p| seen)."""
     import ctypes
@@ -77,6 +77,9 @@ def timed_march(field_np, params, dz, K, Wm, P, net=None, dpdt_max=0.0):
     nX, nY, nT = field_np.shape
     plan = MarchPlan(field_np.shape, params)
     plan.set_dz(dz)
+    if tune is not None:
+        # the launch configuration (radix orders, column grid) chosen by the autotuner
+        tune.update(plan.autotune())
     if net is not None:
         plan.set_learned(net, dpdt_max)
     rows = K + Wm + P + 1
@@ -242,6 +245,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned nonlinear step leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-solve leg")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="march with the default launch configuration instead of the autotuned one")
     ap.add_argument("--batch-instances", type=int, default=32)
     ap.add_argument("--learned-pairs", type=int, default=200_000)
     ap.add_argument("--learned-epochs", type=int, default=3)
@@ -264,7 +269,8 @@ def main():
     vox = nX * nY * nT
     dz = params.dZmin
     K, Wm, P = args.steps, args.warmup, args.profile_iters
-    elapsed, kernels, fin = timed_march(field_np, params, dz, K, Wm, P)
+    tune = None if args.no_autotune else {}
+    elapsed, kernels, fin = timed_march(field_np, params, dz, K, Wm, P, tune=tune)
     value = vox * K / elapsed
     ms_step = elapsed / K * 1e3
     dom = max(kernels, key=lambda n: kernels[n]["avg_ms"] * kernels[n]["launches_per_step"])
@@ -324,7 +330,7 @@ def main():
                                "1 MHz 5 cycles 1 MPa, split-step KT, power-law atten y=1.1, Wendland+"
                                "freq-weighted+super-absorbing boundaries, dZ=dX)",
                    "nX": nX, "nY": nY, "nT": nT, "dZ": dz, "z_steps_per_s": K / elapsed,
-                   "max_abs_p_seen": fin},
+                   "max_abs_p_seen": fin, "launch": tune or "default"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "alg_bytes_per_launch": ab},
