@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
     assert sorted(_lib.EXPORTED) == declared
 
 
-def test_struct_layout_matches_c():
+def test_struct_layout_matches_c(tmp_path):
     from angularspectrum_heterogeneous_amd import _lib
     src = r'''
 #include <stdio.h>
@@ -47,9 +47,8 @@ int main(void) {
   return 0;
 }
 '''
-    tmp = os.path.join(ROOT, "_work") if os.path.isdir(os.path.join(ROOT, "_work")) else "/tmp"
-    c = os.path.join(tmp, "abi_layout.c")
-    exe = os.path.join(tmp, "abi_layout")
+    c = str(tmp_path / "abi_layout.c")
+    exe = str(tmp_path / "abi_layout")
     open(c, "w").write(src)
     subprocess.run(["gcc", "-I", os.path.dirname(HEADER), c, "-o", exe], check=True)
     out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split("\n")
