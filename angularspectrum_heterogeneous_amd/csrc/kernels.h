@@ -327,7 +327,9 @@ __global__ __launch_bounds__(LNT) void k_ypass(float2* __restrict__ spec, Tables
   const int ip = blockIdx.x / nfb, fb = blockIdx.x % nfb;
   const int f0 = fb * LFY;
   const uint32_t syb = (uint32_t)g.MP * 8u;  // y stride in bytes
-  const int nvalid = min(LFY, g.M + 1 - f0);
+  // lines past the Nyquist bin (padding, f > M) are transformed like the others
+  // (no guard branches around the loads and stores): each line is an independent
+  // transform, so padding values never reach a valid bin
   load_tw<L>(tw, T.twy, tid, LNT);
   if (tid < LFY) {
     const int m = min(f0 + tid, g.MP - 1);
@@ -371,7 +373,7 @@ __global__ __launch_bounds__(LNT) void k_ypass(float2* __restrict__ spec, Tables
     const int x = h == 0 ? ip : g.nX - 1 - ip;
     float2* base = spec + (long)x * g.nY * g.MP + f0;
     auto ld = [&](int line, int pos) -> float2 {
-      return line < nvalid ? gld(base, pos * syb + line * 8u) : make_float2(0.f, 0.f);
+      return gld(base, pos * syb + line * 8u);
     };
     auto st_lds = [&](int line, int pos, float2 v) { buf[LY::idx(line, pos)] = v; };
     // pass 0 stores to LDS only after its barrier, which follows every read of
@@ -382,7 +384,7 @@ __global__ __launch_bounds__(LNT) void k_ypass(float2* __restrict__ spec, Tables
       return cmul(buf[LY::idx(line, pos)], opt[i * LFY + line]);
     };
     auto st = [&](int line, int pos, float2 v) {
-      if (line < nvalid) gst(base, pos * syb + line * 8u, v);
+      gst(base, pos * syb + line * 8u, v);
     };
     __syncthreads();
     fft_run<L, true, LNT, LFY, LY, V>(buf, tw, t, ld_op, st);
