@@ -2,7 +2,8 @@
 //
 // State between kernels: the rfft_t half spectrum S[x][y][f] (complex64),
 // f fastest, row length MP (>= M+1, multiple of 16), M = nT/2.  Bin M is the
-// Nyquist bin, bins > M are padding and never written.
+// Nyquist bin, bins > M are padding: only the y pass touches them (it transforms
+// them like any other line), and no valid bin depends on them.
 //
 // Line sweeps (x and y) work on tiles of one line family: all positions along
 // the transform axis for LF=16 consecutive f bins (128-byte segments).  The
