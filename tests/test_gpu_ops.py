@@ -118,7 +118,7 @@ def test_march_step_split_kt(shape):
     assert e < 5e-6
 
 
-@pytest.mark.parametrize("shape", SIZES[:3])
+@pytest.mark.parametrize("shape", SIZES[:3] + SIZES[4:])
 def test_march_step_nonlinearity_obliquity(shape):
     """The composite step with N scaled by the beam obliquity (ASS:956-965)."""
     p = _params(shape, useNonlinearityObliquity=True)
