@@ -355,7 +355,7 @@ __global__ __launch_bounds__(LNT) void k_ypass(float2* __restrict__ spec, Tables
     const double kx = T.kx[ip];
     const double kx2 = kx * kx;
     // entry e = tid + LNT k is on line e % LF = myline
-#pragma unroll 1
+#pragma unroll 2
     for (int e = tid; e < H * LFY; e += LNT) {
       const double ky = T.ky[e / LFY];
       const double kk = kx2 + ky * ky;
