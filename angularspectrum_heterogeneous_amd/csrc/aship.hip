@@ -408,7 +408,8 @@ void launch_xinv_L(ash_plan* p, const EpiDesc& e, hipStream_t s) {
 }
 template <int L>
 void launch_ysa_L(ash_plan* p, const ScreenDesc& sd, hipStream_t s) {
-  hipLaunchKernelGGL(k_ysa<L>, dim3(p->g.nX * p->g.nfb), dim3(LNT), 0, s, p->spec, p->T, p->g, sd);
+  const size_t lds = (size_t)ysa_rows(L, sd.band, sd.phase != nullptr) * LF * sizeof(float2);
+  hipLaunchKernelGGL(k_ysa<L>, dim3(p->g.nX * p->g.nfb), dim3(LNT), lds, s, p->spec, p->T, p->g, sd);
 }
 template <int M, int MODE, int NLK>
 void launch_col_MN(ash_plan* p, const ColDesc& cd, hipStream_t s) {

@@ -496,6 +496,12 @@ __global__ __launch_bounds__(LNT) void k_xfft_inv(float2* __restrict__ spec, Tab
   }
 }
 
+// Rows k_ysa keeps in LDS: all of them with a screen, else the two boundary bands
+// (rows 0..band and L-1-band..L-1).
+__host__ __device__ inline int ysa_rows(int L, int band, bool screen) {
+  return (!screen && 2 * (band + 1) < L) ? 2 * (band + 1) : L;
+}
+
 // ---------------------------------------------------------------------------
 // y-sweep without FFT: super-absorbing y pass, DC/Nyquist projection and an
 // optional phase/amplitude screen.  Tile = (x, f-block).
@@ -512,7 +518,9 @@ struct ScreenDesc {
 
 template <int L>
 __global__ __launch_bounds__(LNT) void k_ysa(float2* __restrict__ spec, Tables T, Geom g, ScreenDesc s) {
-  __shared__ float2 buf[L * LF];
+  // rows handled x LF entries, sized at launch (ysa_rows): the two boundary bands
+  // need a fraction of a full line, and the smaller buffer lets more blocks share a CU
+  extern __shared__ float2 buf[];
   const int tid = threadIdx.x;
   const int x = blockIdx.x / g.nfb, fb = blockIdx.x % g.nfb;
   const int f0 = fb * LF;
@@ -522,8 +530,8 @@ __global__ __launch_bounds__(LNT) void k_ysa(float2* __restrict__ spec, Tables T
   // rows handled: all of them with a screen, else only the two boundary bands
   // (rows 0..band and L-1-band..L-1, which hold every row with a nonzero
   // super-absorbing weight and its neighbours)
-  const bool band = s.phase == nullptr && 2 * (s.band + 1) < L;
-  const int nrows = band ? 2 * (s.band + 1) : L;
+  const int nrows = ysa_rows(L, s.band, s.phase != nullptr);
+  const bool band = nrows < L;
   auto row_of = [&](int i) { return (band && i > s.band) ? L - nrows + i : i; };
   const int nel = nrows * LF;
   for (int el = tid; el < nel; el += LNT) {
