@@ -127,3 +127,12 @@ def test_single_hip_runtime_in_process():
     maps = open("/proc/self/maps").read()
     libs = {l.split()[-1] for l in maps.splitlines() if "libamdhip64" in l}
     assert len(libs) == 1, libs
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus N > 1 exits with a message before any device use."""
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU only" in r.stderr
