@@ -457,3 +457,196 @@ def test_beam_obliquity_scalar_closed_forms():
     assert abs(s - want) < 2e-7
     # scale invariant and >= 1
     assert abs(O.beam_obliquity_scalar(3.7e5 * f, obl) - s) < 1e-12 and s > 1.0
+
+
+# ----------------------------------------------- limiters and TVD scheme ---
+def test_minmod_limiters_match_textbook_forms():
+    """ASS:727-755: two-argument minmod = (sgn a + sgn b)/2 min(|a|, |b|); the
+    three-argument form is the smallest positive value when all are positive, the
+    largest negative value when all are negative, else 0 (elementwise brute force);
+    MC = minmod(2 dm, (dm + dp)/2, 2 dp) on cases worked by hand."""
+    rng = np.random.default_rng(11)
+    a, b, c = (rng.standard_normal(3000) for _ in range(3))
+    a[::7] = 0.0
+    b[::11] = 0.0
+    np.testing.assert_array_equal(O.minmod(a, b),
+                                  0.5 * (np.sign(a) + np.sign(b)) * np.minimum(np.abs(a), np.abs(b)))
+    want = [min(x, y, z) if min(x, y, z) > 0 else (max(x, y, z) if max(x, y, z) < 0 else 0.0)
+            for x, y, z in zip(a, b, c)]
+    np.testing.assert_array_equal(O.minmod3(a, b, c), np.array(want))
+    dm = np.array([1.0, 1.0, 1.0, -2.0, 1.0, 0.0])
+    dp = np.array([1.0, 3.0, 0.25, -1.0, -1.0, 5.0])
+    # equal slopes -> the slope; steep right -> 2 dm; flat right -> 2 dp; negative pair;
+    # extremum -> 0; a zero slope -> 0
+    np.testing.assert_array_equal(O.mc_limiter(dm, dp), [1.0, 2.0, 0.5, -1.5, 0.0, 0.0])
+
+
+@pytest.mark.parametrize("beta", [0.5, 1.0, 1.3, 2.0])
+def test_tvd_rusanov_reduces_to_scaled_minmod_limiter(beta):
+    """ASS:699-724.  The limiter phi(r) = max(0, min(beta r, 1, r, beta)) equals
+    min(beta, 1) clip(r, 0, 1), and the TVD interface flux is half the standard one
+    (ASS:688-696), so the update is u - c/4 min(beta, 1) [clip(r) + clip(1/r)]
+    (F[i+1/2] - F[i-1/2]) with F the standard Rusanov flux.  For beta >= 1, the only
+    values the driver uses (ASS:1465-1466), beta has no effect at all."""
+    rng = np.random.default_rng(5)
+    u = rng.standard_normal((2, 3, 64))
+    N, dZ, dT = 1.0, 0.01, 0.1
+    got = O.rusanov_flux_tvd(u, N, dZ, dT, beta)
+    uL, uR = u[..., :-1], u[..., 1:]
+    F = -(uL ** 2 + uR ** 2) / 2 - np.maximum(np.abs(uL), np.abs(uR)) * (uR - uL)
+    dm = u[..., 1:-1] - u[..., :-2]
+    dp = u[..., 2:] - u[..., 1:-1]
+    phi = min(beta, 1.0) * (np.clip(dp / dm, 0, 1) + np.clip(dm / dp, 0, 1))
+    want = u.copy()
+    want[..., 1:-1] -= N * dZ / dT * 0.25 * phi * (F[..., 1:] - F[..., :-1])
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-13)
+    if beta >= 1.0:
+        np.testing.assert_array_equal(got, O.rusanov_flux_tvd(u, N, dZ, dT, 1.0))
+
+
+def test_tvd_leaves_constants_and_conserves_interior_sum():
+    u = np.full((1, 2, 20), 0.7)
+    np.testing.assert_array_equal(O.rusanov_flux_tvd(u, 1.0, 0.01, 0.1, 1.5), u)
+    rng = np.random.default_rng(2)
+    v = rng.standard_normal((2, 2, 40))
+    v[..., :3] = 0
+    v[..., -3:] = 0
+    # the limiter weights each flux difference separately, so the TVD update is not
+    # in conservation form; a monotone ramp (phi = 1 everywhere) is
+    w = np.concatenate([np.zeros(3), np.linspace(0.1, 1.0, 30), np.ones(3)])[None, None, :]
+    d = O.rusanov_flux_tvd(w, 1.0, 0.01, 0.1, 1.5) - w
+    np.testing.assert_allclose(d.sum(axis=-1), d[..., 1:-1].sum(axis=-1), atol=1e-15)
+
+
+@pytest.mark.parametrize("limiter", ["mc", "minmod"])
+def test_kt_rhs_is_exact_burgers_derivative_on_affine_data(limiter):
+    """ASS:758-830: on affine data both limiters return the exact slope, the two
+    reconstructions meet at every interface (u_L = u_R = w), the central-upwind flux
+    reduces to f(w) = -w^2/2 (to the Lax-Friedrichs branch where w = 0), and the
+    interior rhs is N b u_i / dT exactly (b the slope) -- the Burgers derivative.
+    Integer data keeps every operation exact; the data crosses zero."""
+    b = 2.0
+    u = (-41.0 + b * np.arange(48.0)).reshape(1, 1, -1)
+    N, dT = 1.0, 1.0
+    r = O.kt_rhs(u, N, dT, limiter)
+    np.testing.assert_array_equal(r[0, 0, 2:-2], N * b * u[0, 0, 2:-2] / dT)
+    assert r[0, 0, 0] == 0 and r[0, 0, -1] == 0
+    np.testing.assert_array_equal(O.kt_rhs(u, N, dT, 'mc'), O.kt_rhs(u, N, dT, 'minmod'))
+
+
+def test_kt_minmod_first_order_at_an_extremum_and_conservative():
+    """At a local extremum both limiters give a zero slope, so the two schemes agree
+    on data made only of extrema (alternating signs), and KT-minmod conserves the
+    interior sum on compact data like KT-MC."""
+    u = np.array([0.0, 0.0, 1.0, -2.0, 3.0, -1.0, 2.0, 0.0, 0.0])[None, None, :]
+    np.testing.assert_array_equal(O.kt_rhs(u, 1.0, 0.1, 'minmod'), O.kt_rhs(u, 1.0, 0.1, 'mc'))
+    rng = np.random.default_rng(4)
+    v = rng.standard_normal((3, 2, 50))
+    v[..., :3] = 0
+    v[..., -3:] = 0
+    assert np.max(np.abs(O.kt_rhs(v, 1.0, 0.1, 'minmod').sum(axis=-1))) < 1e-12
+    s = O.kt_flux_minmod(v, 1.0, 0.01, 0.1)
+    np.testing.assert_allclose(s.sum(axis=-1), v.sum(axis=-1), atol=1e-12)
+
+
+# ------------------------------------------------------ water attenuation ---
+def test_water_attenuation_is_the_even_power_law_at_two():
+    """ASS:523-538 against ASS:555-575: the water branch is the power law at y = 2
+    (the even-power Kramers-Kronig branch, tan(pi y / 2) = tan(pi)).  alpha0 is in
+    dB/(cm MHz^2): at f MHz the attenuation in dB/cm is alpha0 f^2; the dispersion
+    term alpha* = alpha tan(pi) (1 - f0/f) is ~1e-16 alpha (no dispersion)."""
+    nT, f0, alpha0 = 64, 1e6, 2.17e-3
+    dT = 1.0 / (nT * 1e6)                     # bin m is m MHz
+    aw, sw = O.water_alpha(alpha0, nT, dT, f0)
+    ap, sp, f, _ = O.powerlaw_alpha(alpha0, nT, dT, f0, 2)
+    np.testing.assert_allclose(aw, ap, rtol=1e-15)
+    np.testing.assert_allclose(sw, sp, rtol=1e-12, atol=0)
+    m = np.arange(nT)
+    np.testing.assert_allclose(aw * 20 * np.log10(np.e) / 1e2, alpha0 * (m * 1.0) ** 2, rtol=1e-12)
+    np.testing.assert_allclose(sw[1:], aw[1:] * np.tan(np.pi) * (1 - f0 / f[1:]), rtol=1e-9, atol=0)
+    assert sw[0] == 0 and np.max(np.abs(sw[1:] / aw[1:])) < 1e-14
+
+
+def test_water_filter_magnitude_and_half_squares_to_full():
+    """ASS:468-549 (water, clipping on): with alpha* ~ 0 the filter is real and
+    positive, exp(-alpha dz k/kz) on propagating modes and 1 on evanescent ones and
+    at DC; the half-step filter squares to the full one."""
+    nX, nY, nT, dX, dZ, dT, c0 = 7, 6, 32, 3e-4, 5e-4, 4e-8, 1482.0
+    af, afh = O.precalculate_ad(2.17e-3, nX, nY, nT, dX, dX, dZ, dT, c0, 1e6, split_step=True,
+                                n_bins=nT // 2 + 1)
+    np.testing.assert_allclose(afh ** 2, af, rtol=1e-12)
+    alpha, _ = O.water_alpha(2.17e-3, nT, dT, 1e6)
+    kx = O.centered_grid(nX, dX)
+    ky = O.centered_grid(nY, dX)
+    kk = kx[:, None] ** 2 + ky[None, :] ** 2
+    assert np.all(af[:, :, 0] == 1)
+    for m in range(1, nT // 2 + 1):
+        k = 2 * np.pi * m / (nT * dT) / c0
+        prop = kk < k * k
+        want = np.where(prop, np.exp(-alpha[m] * dZ * k / np.sqrt(np.where(prop, k * k - kk, 1.0))), 1.0)
+        np.testing.assert_allclose(np.abs(af[:, :, m]), want, rtol=1e-12)
+        assert np.all(np.abs(np.angle(af[:, :, m])) < 1e-12)
+
+
+# ------------------------------------------------- k-space step placement ---
+@pytest.mark.parametrize("shape", [(5, 6, 16), (8, 7, 15)])
+@pytest.mark.parametrize("axis", [0, 1])
+def test_attenuation_step_on_a_transverse_cosine(shape, axis):
+    """ASS:667-685 on u = cos(2 pi j s / n_s) cos(2 pi m t / nT + 0.4) along one
+    transverse axis s (uniform along the other): the power sits on natural bins +j and
+    -j of that axis and bin 0 of the other; fftshift puts natural bin q at index
+    (q + n//2) % n.  The result is Re[(a+ e^{i theta} + a- e^{-i theta}) / 2 e^{i phi}]
+    with a+- the filter at those indices -- for odd and even sizes."""
+    nX, nY, nT = shape
+    rng = np.random.default_rng(nX * nY + axis)
+    nF = nT // 2 + 1
+    af = rng.uniform(0.2, 1.0, (nX, nY, nF)) * np.exp(1j * rng.uniform(-3, 3, (nX, nY, nF)))
+    j, m = 2, 3
+    ns = shape[axis]
+    theta = 2 * np.pi * j * np.arange(ns) / ns
+    phi = 2 * np.pi * m * np.arange(nT) / nT + 0.4
+    prof = np.cos(theta)
+    u = (prof[:, None, None] * np.ones(nY)[None, :, None] if axis == 0
+         else np.ones(nX)[:, None, None] * prof[None, :, None]) * np.cos(phi)[None, None, :]
+    got = O.attenuation_step(u, af)
+    cx, cy = nX // 2, nY // 2
+    if axis == 0:
+        ap, am = af[(j + cx) % nX, cy, m], af[(-j + cx) % nX, cy, m]
+    else:
+        ap, am = af[cx, (j + cy) % nY, m], af[cx, (-j + cy) % nY, m]
+    line = np.real((ap * np.exp(1j * theta) + am * np.exp(-1j * theta))[:, None] / 2
+                   * np.exp(1j * phi)[None, :])
+    want = line[:, None, :] * np.ones(nY)[None, :, None] if axis == 0 else line[None, :, :] * np.ones(nX)[:, None, None]
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-12)
+
+
+def test_freq_weighted_boundary_scales_each_tone_by_its_mask():
+    """ASS:658-664: a DC offset plus one tone at bin m leaves as
+    mask[..., 0] c + mask[..., m] cos(...) -- the mask is applied per t bin."""
+    nX, nY, nT, dT = 12, 10, 64, 4e-8
+    mask = O.precalculate_abl_freq(nX, nY, nT, dT, 1540.0, 1e6, 0.25, 'quadratic')
+    m = 5
+    t = np.arange(nT)
+    tone = np.cos(2 * np.pi * m * t / nT - 1.1)
+    u = np.ones((nX, nY, 1)) * (0.3 + tone)[None, None, :]
+    got = O.freq_weighted_boundary_step(u, mask)
+    want = mask[:, :, :1] * 0.3 + mask[:, :, m:m + 1] * tone[None, None, :]
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-13)
+
+
+def test_kt_rhs_on_quadratic_data_separates_the_limiters():
+    """u = t^2 (t = 1, 2, ...): MC takes the centred slope 2t, so the two
+    reconstructions meet at t^2 + t and the upwind flux f(u_R) gives rhs = 2 N t^3 / dT;
+    minmod takes 2t - 1, the right state at i+1/2 is t^2 + t + 1/2 and
+    rhs = N t (2 t^2 + 1) / dT.  All values are exact in binary."""
+    t = np.arange(1.0, 41.0)
+    u = (t * t)[None, None, :]
+    N, dT = 1.0, 1.0
+    ti = t[2:-2]
+    np.testing.assert_array_equal(O.kt_rhs(u, N, dT, 'mc')[0, 0, 2:-2], 2 * N * ti ** 3 / dT)
+    np.testing.assert_array_equal(O.kt_rhs(u, N, dT, 'minmod')[0, 0, 2:-2], N * ti * (2 * ti ** 2 + 1) / dT)
+    # and kt_flux_minmod is SSP-RK2 over the minmod rhs, not the MC one
+    dz = 1e-3
+    s1 = u + dz * O.kt_rhs(u, N, dT, 'minmod')
+    np.testing.assert_allclose(O.kt_flux_minmod(u, N, dz, dT),
+                               0.5 * u + 0.5 * (s1 + dz * O.kt_rhs(s1, N, dT, 'minmod')), rtol=1e-15)
