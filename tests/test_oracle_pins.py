@@ -650,3 +650,90 @@ def test_kt_rhs_on_quadratic_data_separates_the_limiters():
     s1 = u + dz * O.kt_rhs(u, N, dT, 'minmod')
     np.testing.assert_allclose(O.kt_flux_minmod(u, N, dz, dT),
                                0.5 * u + 0.5 * (s1 + dz * O.kt_rhs(s1, N, dT, 'minmod')), rtol=1e-15)
+
+
+# ------------------------------------------------------ k-filter and abl ----
+@pytest.mark.parametrize("adaptive", [False, True])
+def test_kfilter_is_a_raised_cosine_between_its_edges(adaptive):
+    """ASS:411-433: 1 up to k_start, 0 beyond k_max, the raised cosine
+    (1 + cos(pi s)) / 2 between (1/2 at the midpoint, (2 + sqrt 2)/4 at a quarter),
+    non-increasing in |k|; fixed edges 0.9 * 0.95 kmax .. 0.95 kmax, adaptive edges
+    0.8 k_max .. k_max with k_max <= 0.95 kmax."""
+    dX, dY, dZ, dT, c0 = 3e-4, 2.5e-4, 3e-4, 4e-8, 1540.0
+    kmax = np.pi / dX                                 # min(pi/dX, pi/dY)
+    kt = O.centered_grid(64, dT) / c0
+    kr = np.linspace(0.0, 1.2 * kmax, 20001)
+    filt, (ks, km) = O.kfilter_mask(kr ** 2, dX, dY, dZ, dT, c0, kt, adaptive, 0.05, 0.7)
+    if adaptive:
+        assert km <= 0.95 * kmax and ks == pytest.approx(0.8 * km, rel=1e-15)
+    else:
+        assert km == pytest.approx(0.95 * kmax, rel=1e-15) and ks == pytest.approx(0.9 * km, rel=1e-15)
+    assert np.all(filt[kr <= ks] == 1.0) and np.all(filt[kr > km] == 0.0)
+    assert np.all(np.diff(filt) <= 0)
+    for s, want in ((0.5, 0.5), (0.25, (2 + math.sqrt(2)) / 4), (0.75, (2 - math.sqrt(2)) / 4)):
+        v, _ = O.kfilter_mask(np.array([(ks + s * (km - ks)) ** 2]), dX, dY, dZ, dT, c0, kt, adaptive,
+                              0.05, 0.7)
+        assert v[0] == pytest.approx(want, abs=1e-12)
+
+
+def test_abl_is_separable_with_unit_interior_and_half_squares_to_full():
+    """ASS:116-131: abl(x, y, t) = ax(x) ay(y) at(t), 1 away from the layers, and the
+    split-step half layer is its square root."""
+    nX, nY, nT, bf = 20, 16, 40, 0.2
+    abl, ablh = O.precalculate_abl(nX, nY, nT, bf, split_step=True)
+    wx, wy, wt = (O.boundary_width(n, bf) for n in (nX, nY, nT))
+    assert np.all(abl[wx:nX - wx, wy:nY - wy, wt:nT - wt] == 1.0)
+    # rank one: every (x, y) column is a multiple of the same t profile
+    ref_t = abl[nX // 2, nY // 2]
+    np.testing.assert_allclose(abl, abl[:, :, nT // 2:nT // 2 + 1] * ref_t[None, None, :], rtol=1e-13)
+    np.testing.assert_allclose(ablh ** 2, abl, rtol=1e-13)
+    # the right ramp reaches s = 1 (ASS:88-90), the left one stops at (n - 1)/n: the last
+    # sample of every axis is exactly 0, everything else lies in (0, 1]
+    assert np.all(abl[-1] == 0) and np.all(abl[:, -1] == 0) and np.all(abl[:, :, -1] == 0)
+    assert np.all(abl[:-1, :-1, :-1] > 0) and np.all(abl <= 1)
+
+
+# ----------------------------------------------------------- phase screens ----
+def test_phase_screen_generator_statistics():
+    """ASS:290-349: delta c has the requested standard deviation exactly (it is
+    normalised), the phase inverts to the speed map, and the Gaussian spectral filter
+    exp(-K^2 L^2 / 4) on the power gives the autocorrelation exp(-r^2 / L^2): e^-1 at
+    lag L, e^-4 at 2L (averaged over seeds and both axes)."""
+    n, d, L, c0, f0, sd = 256, 1e-4, 8e-4, 1540.0, 1e6, 25.0
+    r1, r2 = [], []
+    for seed in range(4):
+        ph, cm = O.generate_phase_screen(n, n, d, d, c0, f0, L, sd, thickness=2e-3, seed=seed)
+        dc = cm.astype(np.float64) - c0
+        assert np.std(dc) == pytest.approx(sd, rel=1e-5)
+        c_back = 1.0 / (ph.astype(np.float64) / (2 * np.pi * f0 * 2e-3) + 1.0 / c0)
+        np.testing.assert_allclose(c_back, cm, rtol=1e-6)
+        dc = dc - dc.mean()
+        R = np.real(np.fft.ifft2(np.abs(np.fft.fft2(dc)) ** 2))
+        R /= R[0, 0]
+        lag = int(round(L / d))
+        r1 += [R[lag, 0], R[0, lag]]
+        r2 += [R[2 * lag, 0], R[0, 2 * lag]]
+    assert abs(np.mean(r1) - math.exp(-1)) < 0.05
+    assert abs(np.mean(r2) - math.exp(-4)) < 0.03
+
+
+def test_tone_burst_envelope_and_carrier():
+    """ASS:1195-1200: p0 sin(w0 t) under exp(-(1.05 w0 t / (n pi))^(2 dur)): zero at
+    t = 0 and at the carrier's zero crossings, envelope e^-1 at t = n pi / (1.05 w0),
+    even envelope / odd pulse."""
+    f0, p0, ncyc, dur = 1e6, 2e5, 3, 2
+    w0 = 2 * np.pi * f0
+    t1 = ncyc * np.pi / (1.05 * w0)
+    tq = 1.0 / (4 * f0)                                  # carrier peak, sin = 1
+    assert O.tone_burst(np.array([0.0]), f0, p0, ncyc, dur)[0] == 0.0
+    v = O.tone_burst(np.array([tq]), f0, p0, ncyc, dur)[0]
+    assert v == pytest.approx(p0 * math.exp(-(1.05 * w0 * tq / (ncyc * np.pi)) ** 4), rel=1e-12)
+    k = np.arange(1, 8)
+    np.testing.assert_allclose(O.tone_burst(k / (2 * f0), f0, p0, ncyc, dur), 0.0, atol=1e-9 * p0)
+    # envelope = |pulse| at the carrier peaks nearest t1
+    tp = (np.round(t1 * 2 * f0 - 0.5) + 0.5) / (2 * f0)
+    env = abs(O.tone_burst(np.array([tp]), f0, p0, ncyc, dur)[0]) / p0
+    assert env == pytest.approx(math.exp(-(tp / t1) ** 4), rel=1e-12)
+    t = np.linspace(-5e-6, 5e-6, 101)
+    np.testing.assert_allclose(O.tone_burst(-t, f0, p0, ncyc, dur), -O.tone_burst(t, f0, p0, ncyc, dur),
+                               atol=1e-9 * p0)
