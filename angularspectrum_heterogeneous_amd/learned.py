@@ -4,8 +4,9 @@ The reference has no learned component.  A 4-layer 1-D conv net over t
 (kernel 9, 32 channels, bf16 weights and activations on the matrix cores,
 float32 accumulation) stands in for the KT update over one dZ (ASS:833-839)
 on smooth columns; plain KT stays the default, the fallback wherever a
-column's max|dp/dt| exceeds a threshold, and the judge of the net's error.  Training pairs (column before, column after
-one KT step) come from the plain gfx950 KT kernel itself, and the network is
+column's max|dp/dt| exceeds a threshold, and the yardstick for the net's error.
+Training pairs (column before, column after one KT step) come from the plain
+gfx950 KT kernel itself, and the network is
 trained on the GPU by the library's own forward, backward and Adam kernels.
 
 Scaling: with A = max|u| of a column the net sees x = u / A (features x, x^2)
