@@ -101,6 +101,9 @@ KERNEL_KINDS = ("xfft_fwd", "ypass", "xfft_inv", "ysa", "column_nl", "column_tap
 
 EXPORTED = tuple(_PROTOS)
 
+# ASH_ABI_VERSION of include/aship.h that the prototypes and structures above follow
+ABI_VERSION = 2
+
 _lib: Optional[ctypes.CDLL] = None
 
 
@@ -127,6 +130,10 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = res
+    got = lib.ash_abi_version()
+    if got != ABI_VERSION:
+        raise RuntimeError(f"{path} implements ABI version {got}, the bindings expect {ABI_VERSION}: "
+                           "rebuild it with angularspectrum_heterogeneous_amd._build.build()")
     _lib = lib
     return lib
 

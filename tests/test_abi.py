@@ -29,6 +29,15 @@ def test_library_exports_every_declared_symbol():
     assert sorted(_lib.EXPORTED) == declared
 
 
+def test_abi_version_of_header_bindings_and_library_agree():
+    """ash_abi_version is a host-only call: the loaded library, the header's
+    ASH_ABI_VERSION and the bindings' ABI_VERSION must all be the same number."""
+    from angularspectrum_heterogeneous_amd import _lib
+    m = re.search(r"#define\s+ASH_ABI_VERSION\s+(\d+)", open(HEADER).read())
+    assert m and int(m.group(1)) == _lib.ABI_VERSION
+    assert _lib.load().ash_abi_version() == _lib.ABI_VERSION
+
+
 def test_struct_layout_matches_c(tmp_path):
     from angularspectrum_heterogeneous_amd import _lib
     src = r'''
