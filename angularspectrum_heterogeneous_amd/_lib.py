@@ -14,6 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libaship.so")
 
 ASH_OK = 0
+ASH_E_ARG, ASH_E_UNSUPPORTED, ASH_E_HIP, ASH_E_STATE, ASH_E_HOST = -1, -2, -3, -4, -5
 NL_NONE, NL_KT, NL_RUSANOV, NL_TVD, NL_KT_MINMOD = 0, 1, 2, 3, 4
 SUBCYCLE_OFF, SUBCYCLE_COLUMN = 0, 1
 PROFILE_QUADRATIC, PROFILE_WENDLAND = 0, 1

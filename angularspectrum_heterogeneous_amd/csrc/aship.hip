@@ -7,6 +7,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
+#include <memory>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -23,6 +26,30 @@ thread_local std::string g_err;
 int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
+}
+
+void set_error_nothrow(const char* msg) noexcept {
+  try {
+    g_err = msg;
+  } catch (...) {
+    g_err.clear();
+  }
+}
+
+// Every int-returning entry point is a function-try-block ending in this handler:
+// no C++ exception (std::bad_alloc from a table or a graph, a throwing library call)
+// may unwind through the C ABI into a foreign caller.
+int caught_exception() noexcept {
+  try {
+    throw;
+  } catch (const std::bad_alloc&) {
+    set_error_nothrow("host memory allocation failed");
+  } catch (const std::exception& e) {
+    set_error_nothrow(e.what());
+  } catch (...) {
+    set_error_nothrow("unknown C++ exception");
+  }
+  return ASH_E_HOST;
 }
 
 #define HIP_TRY(expr)                                                                   \
@@ -143,9 +170,10 @@ namespace {
 template <class T>
 int upload(ash_plan* p, const std::vector<T>& h, const T** out) {
   void* d = nullptr;
+  p->allocs.reserve(p->allocs.size() + 1);  // the push_back below cannot throw
   HIP_TRY(hipMalloc(&d, h.size() * sizeof(T) + 16));
-  HIP_TRY(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
   p->allocs.push_back(d);
+  HIP_TRY(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
   p->ws_bytes += (int64_t)(h.size() * sizeof(T));
   *out = (const T*)d;
   return ASH_OK;
@@ -711,7 +739,7 @@ int ash_abi_version(void) { return ASH_ABI_VERSION; }
 
 const char* ash_last_error(void) { return g_err.c_str(); }
 
-int ash_plan_create(const ash_desc* desc, ash_plan** out) {
+int ash_plan_create(const ash_desc* desc, ash_plan** out) try {
   if (!desc || !out) return fail(ASH_E_ARG, "null argument");
   *out = nullptr;
   const ash_desc& d = *desc;
@@ -722,7 +750,9 @@ int ash_plan_create(const ash_desc* desc, ash_plan** out) {
     return fail(ASH_E_UNSUPPORTED, "nX and nY must be one of 64, 128, 192, 256, 384");
   if (d.nT % 2 != 0 || !col_supported(d.nT / 2))
     return fail(ASH_E_UNSUPPORTED, "nT must be one of 256, 512, 1024, 2048");
-  ash_plan* p = new ash_plan();
+  // destroys the partial plan on every early return and on an exception
+  std::unique_ptr<ash_plan, void (*)(ash_plan*)> guard(new ash_plan(), ash_plan_destroy);
+  ash_plan* p = guard.get();
   p->d = d;
   if (p->d.cflTarget <= 0) p->d.cflTarget = 0.5;
   p->g.nX = d.nX; p->g.nY = d.nY; p->g.nT = d.nT;
@@ -733,34 +763,33 @@ int ash_plan_create(const ash_desc* desc, ash_plan** out) {
   if (const char* ev = std::getenv("ASH_PRECISE_OPS")) p->precise_ops = std::atoi(ev);
   p->dz = 0.0;
   int rc = build_tables(p);
-  if (rc) { ash_plan_destroy(p); return rc; }
+  if (rc) return rc;
   const size_t nspec = (size_t)d.nX * d.nY * p->g.MP;
   // the line sweeps address the spectrum with 32-bit byte offsets
-  if (nspec * sizeof(float2) >= (size_t(1) << 32)) {
-    ash_plan_destroy(p);
+  if (nspec * sizeof(float2) >= (size_t(1) << 32))
     return fail(ASH_E_UNSUPPORTED, "spectrum larger than 4 GiB");
-  }
+  // each buffer is recorded as soon as it exists, so a later failure frees it
+  auto dalloc = [p](void** ptr, size_t bytes) -> hipError_t {
+    p->allocs.reserve(p->allocs.size() + 1);
+    hipError_t e = hipMalloc(ptr, bytes);
+    if (e == hipSuccess) p->allocs.push_back(*ptr);
+    return e;
+  };
   hipError_t e;
-  if ((e = hipMalloc(&p->spec, nspec * sizeof(float2))) != hipSuccess ||
-      (e = hipMalloc(&p->ibefore, (size_t)d.nX * d.nY * sizeof(double))) != hipSuccess ||
-      (e = hipMalloc(&p->d_row, 64)) != hipSuccess ||
-      (e = hipMalloc(&p->obl_acc, 64)) != hipSuccess) {
-    ash_plan_destroy(p);
+  if ((e = dalloc((void**)&p->spec, nspec * sizeof(float2))) != hipSuccess ||
+      (e = dalloc((void**)&p->ibefore, (size_t)d.nX * d.nY * sizeof(double))) != hipSuccess ||
+      (e = dalloc((void**)&p->d_row, 64)) != hipSuccess ||
+      (e = dalloc((void**)&p->obl_acc, 64)) != hipSuccess)
     return fail(ASH_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
-  }
-  p->allocs.push_back(p->spec);
-  p->allocs.push_back(p->ibefore);
-  p->allocs.push_back(p->d_row);
-  p->allocs.push_back(p->obl_acc);
   // padding bins are read (masked) but never written: clear once
   if ((e = hipMemset(p->spec, 0, nspec * sizeof(float2))) != hipSuccess ||
-      (e = hipMemset(p->d_row, 0, 64)) != hipSuccess || (e = hipDeviceSynchronize()) != hipSuccess) {
-    ash_plan_destroy(p);
+      (e = hipMemset(p->d_row, 0, 64)) != hipSuccess || (e = hipDeviceSynchronize()) != hipSuccess)
     return fail(ASH_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
-  }
   p->ws_bytes += (int64_t)(nspec * sizeof(float2) + (size_t)d.nX * d.nY * sizeof(double));
-  *out = p;
+  *out = guard.release();
   return ASH_OK;
+} catch (...) {
+  return caught_exception();
 }
 
 void ash_plan_destroy(ash_plan* p) {
@@ -773,26 +802,32 @@ void ash_plan_destroy(ash_plan* p) {
   delete p;
 }
 
-int ash_plan_set_dz(ash_plan* p, double dz) {
+int ash_plan_set_dz(ash_plan* p, double dz) try {
   if (!p) return fail(ASH_E_ARG, "null plan");
   if (!(dz > 0)) return fail(ASH_E_ARG, "dz must be positive");
   p->dz = dz;
   return ASH_OK;
+} catch (...) {
+  return caught_exception();
 }
 
-int ash_load_field(ash_plan* p, const float* field, uint32_t* stats, void* stream) {
+int ash_load_field(ash_plan* p, const float* field, uint32_t* stats, void* stream) try {
   if (!p || !field) return fail(ASH_E_ARG, "null argument");
   hipStream_t s = (hipStream_t)stream;
   load_real(p, field, stats, p->ibefore, s);
   p->loaded = true;
   return check_launch();
+} catch (...) {
+  return caught_exception();
 }
 
-int ash_store_field(ash_plan* p, float* field, void* stream) {
+int ash_store_field(ash_plan* p, float* field, void* stream) try {
   if (!p || !field) return fail(ASH_E_ARG, "null argument");
   if (!p->loaded) return fail(ASH_E_STATE, "no field loaded");
   store_real(p, field, (hipStream_t)stream);
   return check_launch();
+} catch (...) {
+  return caught_exception();
 }
 
 namespace {
@@ -823,7 +858,7 @@ bool same_io(const ash_step_io& a, const ash_step_io& b) {
 }
 }  // namespace
 
-int ash_step(ash_plan* p, const ash_step_io* io, void* stream) {
+int ash_step(ash_plan* p, const ash_step_io* io, void* stream) try {
   int rc = check_io(p, io);
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
@@ -851,9 +886,11 @@ int ash_step(ash_plan* p, const ash_step_io* io, void* stream) {
   }
   p->host_row = io->row + 1;
   return check_launch();
+} catch (...) {
+  return caught_exception();
 }
 
-int ash_plan_autotune(ash_plan* p, int32_t* choices, void* stream) {
+int ash_plan_autotune(ash_plan* p, int32_t* choices, void* stream) try {
   if (!p) return fail(ASH_E_ARG, "null plan");
   hipStream_t s = (hipStream_t)stream;
   const size_t nbytes = (size_t)p->g.nX * p->g.nY * p->g.MP * sizeof(float2);
@@ -929,9 +966,11 @@ int ash_plan_autotune(ash_plan* p, int32_t* choices, void* stream) {
   if (choices) { choices[0] = p->rvx; choices[1] = p->rvy; choices[2] = p->col_grid; }
   if (rc) return rc;
   return check_launch();
+} catch (...) {
+  return caught_exception();
 }
 
-int ash_plan_set_launch(ash_plan* p, const int32_t* choices) {
+int ash_plan_set_launch(ash_plan* p, const int32_t* choices) try {
   if (!p || !choices) return fail(ASH_E_ARG, "null argument");
   if (choices[0] < 0 || choices[0] > 1 || choices[1] < 0 || choices[1] > 1)
     return fail(ASH_E_ARG, "radix orders are 0 or 1");
@@ -943,12 +982,16 @@ int ash_plan_set_launch(ash_plan* p, const int32_t* choices) {
   if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
   if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
   return ASH_OK;
+} catch (...) {
+  return caught_exception();
 }
 
-int ash_batch_create(ash_batch** out) {
+int ash_batch_create(ash_batch** out) try {
   if (!out) return fail(ASH_E_ARG, "null argument");
   *out = new ash_batch();
   return ASH_OK;
+} catch (...) {
+  return caught_exception();
 }
 
 void ash_batch_destroy(ash_batch* b) {
@@ -959,7 +1002,7 @@ void ash_batch_destroy(ash_batch* b) {
   delete b;
 }
 
-int ash_batch_step(ash_batch* b, ash_plan* const* plans, const ash_step_io* ios, int32_t n, void* stream) {
+int ash_batch_step(ash_batch* b, ash_plan* const* plans, const ash_step_io* ios, int32_t n, void* stream) try {
   if (!b || !plans || !ios || n < 1) return fail(ASH_E_ARG, "bad argument");
   bool all_plain = true;
   for (int i = 0; i < n; ++i) {
@@ -999,9 +1042,11 @@ int ash_batch_step(ash_batch* b, ash_plan* const* plans, const ash_step_io* ios,
   }
   for (int i = 0; i < n; ++i) plans[i]->host_row = ios[i].row + 1;
   return check_launch();
+} catch (...) {
+  return caught_exception();
 }
 
-int ash_march_step(ash_plan* p, const float* in, float* out, void* stream) {
+int ash_march_step(ash_plan* p, const float* in, float* out, void* stream) try {
   if (!p || !in || !out) return fail(ASH_E_ARG, "null argument");
   if (!(p->dz > 0)) return fail(ASH_E_STATE, "dz not set");
   hipStream_t s = (hipStream_t)stream;
@@ -1011,9 +1056,11 @@ int ash_march_step(ash_plan* p, const float* in, float* out, void* stream) {
   store_real(p, out, s);
   p->loaded = true;
   return check_launch();
+} catch (...) {
+  return caught_exception();
 }
 
-int ash_attenuation_step(ash_plan* p, const float* in, float* out, double scale, void* stream) {
+int ash_attenuation_step(ash_plan* p, const float* in, float* out, double scale, void* stream) try {
   if (!p || !in || !out) return fail(ASH_E_ARG, "null argument");
   if (!(p->dz > 0)) return fail(ASH_E_STATE, "dz not set");
   hipStream_t s = (hipStream_t)stream;
@@ -1024,9 +1071,11 @@ int ash_attenuation_step(ash_plan* p, const float* in, float* out, double scale,
   store_real(p, out, s);
   p->loaded = true;
   return check_launch();
+} catch (...) {
+  return caught_exception();
 }
 
-int ash_angular_spectrum_step(ash_plan* p, const float* in, float* out, int32_t half, void* stream) {
+int ash_angular_spectrum_step(ash_plan* p, const float* in, float* out, int32_t half, void* stream) try {
   if (!p || !in || !out) return fail(ASH_E_ARG, "null argument");
   if (!(p->dz > 0)) return fail(ASH_E_STATE, "dz not set");
   hipStream_t s = (hipStream_t)stream;
@@ -1043,10 +1092,12 @@ int ash_angular_spectrum_step(ash_plan* p, const float* in, float* out, int32_t 
   store_real(p, out, s);
   p->loaded = true;
   return check_launch();
+} catch (...) {
+  return caught_exception();
 }
 
 int ash_nonlinear_step(ash_plan* p, const float* in, float* out, int32_t scheme, double dz, double beta,
-                       int32_t subcycle, void* stream) {
+                       int32_t subcycle, void* stream) try {
   if (!p || !in || !out) return fail(ASH_E_ARG, "null argument");
   if (scheme < ASH_NL_NONE || scheme > ASH_NL_KT_MINMOD) return fail(ASH_E_ARG, "unknown scheme");
   ColDesc cd = col_base(p);
@@ -1059,18 +1110,22 @@ int ash_nonlinear_step(ash_plan* p, const float* in, float* out, int32_t scheme,
   cd.rout = out;
   column(p, cd, (hipStream_t)stream);
   return check_launch();
+} catch (...) {
+  return caught_exception();
 }
 
-int ash_max_abs(ash_plan* p, const float* in, uint32_t* out_bits, void* stream) {
+int ash_max_abs(ash_plan* p, const float* in, uint32_t* out_bits, void* stream) try {
   if (!p || !in || !out_bits) return fail(ASH_E_ARG, "null argument");
   // the load kernel computes max|p| of its input; the spectrum it writes is
   // the plan state, so this call also loads `in` as the current field
   load_real(p, in, out_bits, nullptr, (hipStream_t)stream);
   p->loaded = true;
   return check_launch();
+} catch (...) {
+  return caught_exception();
 }
 
-int ash_freq_weighted_boundary_step(ash_plan* p, const float* in, float* out, void* stream) {
+int ash_freq_weighted_boundary_step(ash_plan* p, const float* in, float* out, void* stream) try {
   if (!p || !in || !out) return fail(ASH_E_ARG, "null argument");
   hipStream_t s = (hipStream_t)stream;
   load_real(p, in, nullptr, nullptr, s);
@@ -1082,9 +1137,11 @@ int ash_freq_weighted_boundary_step(ash_plan* p, const float* in, float* out, vo
   store_real(p, out, s);
   p->loaded = true;
   return check_launch();
+} catch (...) {
+  return caught_exception();
 }
 
-int ash_super_absorbing_step(ash_plan* p, const float* in, float* out, void* stream) {
+int ash_super_absorbing_step(ash_plan* p, const float* in, float* out, void* stream) try {
   if (!p || !in || !out) return fail(ASH_E_ARG, "null argument");
   hipStream_t s = (hipStream_t)stream;
   load_real(p, in, nullptr, nullptr, s);
@@ -1104,10 +1161,12 @@ int ash_super_absorbing_step(ash_plan* p, const float* in, float* out, void* str
   store_real(p, out, s);
   p->loaded = true;
   return check_launch();
+} catch (...) {
+  return caught_exception();
 }
 
 int ash_apply_phase_screen(ash_plan* p, const float* in, float* out, const float* phase, const float* amp,
-                           void* stream) {
+                           void* stream) try {
   if (!p || !in || !out || !phase) return fail(ASH_E_ARG, "null argument");
   hipStream_t s = (hipStream_t)stream;
   load_real(p, in, nullptr, nullptr, s);
@@ -1121,10 +1180,12 @@ int ash_apply_phase_screen(ash_plan* p, const float* in, float* out, const float
   store_real(p, out, s);
   p->loaded = true;
   return check_launch();
+} catch (...) {
+  return caught_exception();
 }
 
 int ash_source_field(const uint8_t* mask, const double* delay, const double* taxis, int64_t ncols, int32_t nT,
-                     double f0, double p0, double ncycles, double dur, float* out, void* stream) {
+                     double f0, double p0, double ncycles, double dur, float* out, void* stream) try {
   if (!mask || !delay || !taxis || !out) return fail(ASH_E_ARG, "null argument");
   if (ncols <= 0 || nT <= 0) return fail(ASH_E_ARG, "ncols and nT must be positive");
   if (!(ncycles != 0.0)) return fail(ASH_E_ARG, "ncycles must be nonzero");
@@ -1133,10 +1194,12 @@ int ash_source_field(const uint8_t* mask, const double* delay, const double* tax
   hipLaunchKernelGGL(k_source_field, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, mask, delay, taxis,
                      (long)ncols, (int)nT, 2.0 * M_PI * f0, ncycles, 2.0 * dur, p0, out);
   return check_launch();
+} catch (...) {
+  return caught_exception();
 }
 
 int ash_profile_step(ash_plan* p, const ash_step_io* io, int32_t n_iter, float* ms_out, int32_t* count_out,
-                     void* stream) {
+                     void* stream) try {
   if (!p || !io || !ms_out || !count_out || n_iter <= 0) return fail(ASH_E_ARG, "bad argument");
   if (!p->loaded) return fail(ASH_E_STATE, "ash_profile_step before ash_load_field");
   if (io->row < 0 || io->row + n_iter > io->max_rows) return fail(ASH_E_ARG, "rows out of range");
@@ -1163,6 +1226,8 @@ int ash_profile_step(ash_plan* p, const ash_step_io* io, int32_t n_iter, float* 
     count_out[k] /= n_iter;  // launches per step
   }
   return ASH_OK;
+} catch (...) {
+  return caught_exception();
 }
 
 int64_t ash_plan_workspace_bytes(const ash_plan* p) { return p ? p->ws_bytes : 0; }
@@ -1172,19 +1237,22 @@ int64_t ash_plan_workspace_bytes(const ash_plan* p) { return p ? p->ws_bytes : 0
 // ---------------------------------------------------------------------------
 int64_t ash_net_param_count(void) { return net::PARAM_COUNT; }
 
-int ash_net_create(int32_t nT, int32_t max_cols, ash_net** out) {
+int ash_net_create(int32_t nT, int32_t max_cols, ash_net** out) try {
   if (!out) return fail(ASH_E_ARG, "null argument");
   *out = nullptr;
   if (!(nT == 256 || nT == 512 || nT == 1024 || nT == 2048))
     return fail(ASH_E_UNSUPPORTED, "network column length must be 256, 512, 1024 or 2048");
   if (max_cols < 1) return fail(ASH_E_ARG, "max_cols must be positive");
-  ash_net* n = new ash_net();
+  // destroys the partial network on every early return and on an exception
+  std::unique_ptr<ash_net, void (*)(ash_net*)> guard(new ash_net(), ash_net_destroy);
+  ash_net* n = guard.get();
   n->nT = nT;
   n->max_cols = max_cols;
   int dev = 0;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&n->n_cu, hipDeviceAttributeMultiprocessorCount, dev);
   auto alloc = [&](void** ptr, size_t bytes) -> int {
+    n->allocs.reserve(n->allocs.size() + 1);  // the push_back below cannot throw
     hipError_t e = hipMalloc(ptr, bytes);
     if (e != hipSuccess) return fail(ASH_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
     n->allocs.push_back(*ptr);
@@ -1217,9 +1285,11 @@ int ash_net_create(int32_t nT, int32_t max_cols, ash_net** out) {
     hipError_t e = hipMemcpy(n->mask, mk.data(), np * 4, hipMemcpyHostToDevice);
     if (e != hipSuccess) rc = fail(ASH_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
   }
-  if (rc) { ash_net_destroy(n); return rc; }
-  *out = n;
+  if (rc) return rc;
+  *out = guard.release();
   return ASH_OK;
+} catch (...) {
+  return caught_exception();
 }
 
 void ash_net_destroy(ash_net* n) {
@@ -1228,19 +1298,23 @@ void ash_net_destroy(ash_net* n) {
   delete n;
 }
 
-int ash_net_set_params(ash_net* n, const float* params, void* stream) {
+int ash_net_set_params(ash_net* n, const float* params, void* stream) try {
   if (!n || !params) return fail(ASH_E_ARG, "null argument");
   hipStream_t s = (hipStream_t)stream;
   HIP_TRY(hipMemcpyAsync(n->params, params, net::PARAM_COUNT * 4, hipMemcpyDeviceToDevice, s));
   hipLaunchKernelGGL(net::k_net_pack, dim3((net::NLAY * net::W_SIZE + 255) / 256), dim3(256), 0, s, n->params, n->Wm,
                      n->Wt);
   return check_launch();
+} catch (...) {
+  return caught_exception();
 }
 
-int ash_net_get_params(ash_net* n, float* params, void* stream) {
+int ash_net_get_params(ash_net* n, float* params, void* stream) try {
   if (!n || !params) return fail(ASH_E_ARG, "null argument");
   HIP_TRY(hipMemcpyAsync(params, n->params, net::PARAM_COUNT * 4, hipMemcpyDeviceToDevice, (hipStream_t)stream));
   return ASH_OK;
+} catch (...) {
+  return caught_exception();
 }
 
 namespace {
@@ -1258,14 +1332,16 @@ int net_forward(ash_net* n, const float* u, int ncols, float* y, float* amp, hip
 }
 }  // namespace
 
-int ash_net_forward(ash_net* n, const float* u, int32_t ncols, float* y, float* amp, void* stream) {
+int ash_net_forward(ash_net* n, const float* u, int32_t ncols, float* y, float* amp, void* stream) try {
   if (!n || !u || !y || !amp) return fail(ASH_E_ARG, "null argument");
   if (ncols < 1 || ncols > n->max_cols) return fail(ASH_E_ARG, "ncols out of range");
   return net_forward(n, u, ncols, y, amp, (hipStream_t)stream);
+} catch (...) {
+  return caught_exception();
 }
 
 int ash_net_loss_grad(ash_net* n, const float* u, const float* un, int32_t ncols, double c, double* loss,
-                      float* grads, void* stream) {
+                      float* grads, void* stream) try {
   if (!n || !u || !un || !loss || !grads) return fail(ASH_E_ARG, "null argument");
   if (ncols < 1 || ncols > n->max_cols) return fail(ASH_E_ARG, "ncols out of range");
   if (!(c > 0)) return fail(ASH_E_ARG, "c must be positive");
@@ -1288,10 +1364,12 @@ int ash_net_loss_grad(ash_net* n, const float* u, const float* un, int32_t ncols
     }
   }
   return check_launch();
+} catch (...) {
+  return caught_exception();
 }
 
 int ash_net_adam(ash_net* n, const float* grads, float lr, float b1, float b2, float eps, int32_t step,
-                 void* stream) {
+                 void* stream) try {
   if (!n || !grads) return fail(ASH_E_ARG, "null argument");
   if (step < 1) return fail(ASH_E_ARG, "step counts from 1");
   hipStream_t s = (hipStream_t)stream;
@@ -1301,19 +1379,23 @@ int ash_net_adam(ash_net* n, const float* grads, float lr, float b1, float b2, f
   hipLaunchKernelGGL(net::k_net_pack, dim3((net::NLAY * net::W_SIZE + 255) / 256), dim3(256), 0, s, n->params, n->Wm,
                      n->Wt);
   return check_launch();
+} catch (...) {
+  return caught_exception();
 }
 
 int ash_net_apply(ash_net* n, float* u, int32_t ncols, double c, double dpdt_max, double dT, int32_t* flag,
-                  void* stream) {
+                  void* stream) try {
   if (!n || !u || !flag) return fail(ASH_E_ARG, "null argument");
   if (ncols < 1) return fail(ASH_E_ARG, "ncols must be positive");
   if (!(dT > 0)) return fail(ASH_E_ARG, "dT must be positive");
   int rc = net_apply(n, u, ncols, c, dpdt_max, dT, flag, (hipStream_t)stream);
   if (rc) return rc;
   return check_launch();
+} catch (...) {
+  return caught_exception();
 }
 
-int ash_plan_set_learned(ash_plan* p, ash_net* n, double dpdt_max) {
+int ash_plan_set_learned(ash_plan* p, ash_net* n, double dpdt_max) try {
   if (!p) return fail(ASH_E_ARG, "null plan");
   if (n) {
     if (!p->d.useSplitStep || p->d.fluxScheme != 1)
@@ -1322,13 +1404,18 @@ int ash_plan_set_learned(ash_plan* p, ash_net* n, double dpdt_max) {
       return fail(ASH_E_UNSUPPORTED, "the learned step does not take the beam obliquity scale");
     if (n->nT != p->d.nT) return fail(ASH_E_ARG, "network column length differs from the plan's nT");
     const size_t ncol = (size_t)p->d.nX * p->d.nY;
+    // each buffer is allocated while missing, so a call after a failed one completes the set
+    p->allocs.reserve(p->allocs.size() + 2);  // the push_backs below cannot throw
     if (!p->rscratch) {
       HIP_TRY(hipMalloc(&p->rscratch, ncol * p->d.nT * sizeof(float)));
       p->allocs.push_back(p->rscratch);
+      p->ws_bytes += (int64_t)(ncol * p->d.nT * sizeof(float));
+    }
+    if (!p->nflag) {
       HIP_TRY(hipMalloc(&p->nflag, ncol * sizeof(int32_t)));
       p->allocs.push_back(p->nflag);
+      p->ws_bytes += (int64_t)(ncol * sizeof(int32_t));
       HIP_TRY(hipMemset(p->nflag, 0, ncol * sizeof(int32_t)));
-      p->ws_bytes += (int64_t)(ncol * p->d.nT * sizeof(float) + ncol * sizeof(int32_t));
     }
   }
   p->net = n;
@@ -1338,6 +1425,8 @@ int ash_plan_set_learned(ash_plan* p, ash_net* n, double dpdt_max) {
   if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
   if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
   return ASH_OK;
+} catch (...) {
+  return caught_exception();
 }
 
 }  // extern "C"

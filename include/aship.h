@@ -37,6 +37,7 @@ extern "C" {
 #define ASH_E_UNSUPPORTED (-2) /* grid size or option not implemented */
 #define ASH_E_HIP (-3)         /* HIP runtime error */
 #define ASH_E_STATE (-4)       /* call out of order (e.g. step before load) */
+#define ASH_E_HOST (-5)        /* host-side failure: out of host memory, internal exception */
 
 /* boundaryProfile */
 #define ASH_PROFILE_QUADRATIC 0

@@ -82,6 +82,34 @@ def test_plan_create_rejects_unsupported_sizes_before_device_use():
         d = desc_from_params(shape, p)
         assert lib.ash_plan_create(ctypes.byref(d), ctypes.byref(h)) == -2
         assert b"must be" in lib.ash_last_error()
+
+
+def test_error_codes_match_header():
+    from angularspectrum_heterogeneous_amd import _lib
+    txt = open(HEADER).read()
+    for name in ("ASH_E_ARG", "ASH_E_UNSUPPORTED", "ASH_E_HIP", "ASH_E_STATE", "ASH_E_HOST"):
+        m = re.search(r"#define\s+%s\s+\((-\d+)\)" % name, txt)
+        assert m and int(m.group(1)) == getattr(_lib, name), name
+
+
+def test_create_failing_on_the_device_returns_an_error_and_no_handle():
+    """Without a device the first hipMalloc of a valid plan or network fails: the
+    call returns ASH_E_HIP with the runtime's text, leaves the handle NULL and frees
+    the partial object (no crash, nothing escapes the C ABI)."""
+    import torch
+    if torch.cuda.device_count() > 0:
+        pytest.skip("a GPU is present: creation succeeds here")
+    from angularspectrum_heterogeneous_amd import _lib
+    from angularspectrum_heterogeneous_amd.plan import desc_from_params
+    from angularspectrum_heterogeneous_amd.solver import SolverParams
+    lib = _lib.load()
+    d = desc_from_params((64, 64, 256), SolverParams(dX=3e-4, dY=3e-4, dT=4e-8))
+    h = ctypes.c_void_p(1)
+    assert lib.ash_plan_create(ctypes.byref(d), ctypes.byref(h)) == _lib.ASH_E_HIP
+    assert h.value is None and b"hipMalloc" in lib.ash_last_error()
+    n = ctypes.c_void_p(1)
+    assert lib.ash_net_create(256, 16, ctypes.byref(n)) == _lib.ASH_E_HIP
+    assert n.value is None and b"hipMalloc" in lib.ash_last_error()
     d = desc_from_params((64, 64, 256), SolverParams(dX=0.0, dY=3e-4, dT=4e-8))
     assert lib.ash_plan_create(ctypes.byref(d), ctypes.byref(h)) == -1
     assert lib.ash_plan_create(None, ctypes.byref(h)) == -1
