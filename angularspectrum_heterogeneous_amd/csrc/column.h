@@ -12,6 +12,9 @@
 
 namespace ash {
 
+// Most KT sub-steps one column takes in one dZ with per-column CFL sub-cycling
+constexpr int MAX_SUBSTEPS = 1024;
+
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -251,7 +254,10 @@ __global__ __launch_bounds__(256) void k_column(float2* __restrict__ spec, Table
           for (int i = 0; i < S; ++i) mx = fmaxf(mx, fabsf(u[i]));
           for (int o = LPC / 2; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
           const double cfl = Nn * (double)mx * d.dz / d.dT;
-          if (!(cfl <= (double)d.cfl_target || mx < 1e-30f)) nsub = (int)ceil(cfl / (double)d.cfl_target);
+          // ceil(cfl / target) sub-steps above the target, bounded so that a blown-up
+          // column (inf) cannot keep the wave looping; NaN takes one step
+          const double q = cfl / (double)d.cfl_target;
+          if (q > 1.0 && mx >= 1e-30f) nsub = q < (double)MAX_SUBSTEPS ? (int)ceil(q) : MAX_SUBSTEPS;
         }
         int nsub_w = nsub;
         for (int o = 32; o >= LPC; o >>= 1) nsub_w = max(nsub_w, __shfl_xor(nsub_w, o, 64));
