@@ -20,8 +20,12 @@ reference's jitted kernels use ``jax.numpy.fft`` (XLA/ducc); both are exact DFTs
 up to rounding, and both c2r transforms discard the imaginary part of the DC and
 Nyquist bins, which the pipeline relies on (checked in the tests).
 
-Pinning status of each function is recorded in the test files
-(``tests/test_oracle_*.py``); functions pinned only by restatement say so there.
+Each function is pinned in ``tests/test_oracle_pins.py`` by checks that do not
+retype its formula: closed forms, invariants, the reference's known-answer test and
+published convergence orders.  Pinned only by restatement: the scheme dispatch
+(``nonlinear_kind``, ``beta_tvd_for``, ASS:1454-1492) and the place of the nonlinear
+update inside the sequential step (ASS:874-880); the linear part of every composite
+step is pinned.
 """
 from __future__ import annotations
 

@@ -737,3 +737,39 @@ def test_tone_burst_envelope_and_carrier():
     t = np.linspace(-5e-6, 5e-6, 101)
     np.testing.assert_allclose(O.tone_burst(-t, f0, p0, ncyc, dur), -O.tone_burst(t, f0, p0, ncyc, dur),
                                atol=1e-9 * p0)
+
+
+# ------------------------------------------------------ composite steps -----
+@pytest.mark.parametrize("alpha0", [0.5, -1.0])
+def test_sequential_and_strang_steps_agree_when_linear(alpha0):
+    """ASS:874-880 against ASS:905-913 with beta = 0 and no absorbing layer: both
+    steps reduce to their linear k-space operators, each diagonal in the same DFT basis
+    (real-part projections included), so they commute and the Lie step A H must equal
+    the Strang step A^1/2 H^1/2 H^1/2 A^1/2 -- for a field whose transverse spectrum lies
+    inside the k-filter's flat region (the filter enters the full and the half
+    propagator once each).  Power law and water attenuation."""
+    from angularspectrum_heterogeneous_amd import workloads as W
+    nX = nY = 32
+    nT = 128
+    # band-limited across x and y: three low-index cosines (natural bins +-j), far from
+    # the filter's roll-off at 0.855 kmax; a sampled Gaussian would leak into it
+    x, y = np.arange(nX)[:, None], np.arange(nY)[None, :]
+    pattern = (1.0 + 0.6 * np.cos(2 * np.pi * (2 * x / nX + 1 * y / nY) + 0.3)
+               + 0.4 * np.cos(2 * np.pi * (3 * y / nY) - 1.1) + 0.3 * np.cos(2 * np.pi * 4 * x / nX))
+    t = W.axes(nT, 4e-8)
+    field = pattern[:, :, None] * W.tone_burst(t, 1e6, 1e5, 3)[None, None, :]
+    kw = W.params_kwargs(dX=3.08e-4, dY=3.08e-4, dT=4e-8, propDist=2e-3, dZmin=3.08e-4, beta=0.0,
+                         alpha0=alpha0, useBoundaryLayer=False, useAdaptiveFiltering=False)
+    dz = 6e-4
+    p_seq = O.OracleParams(**dict(kw, useSplitStep=False))
+    p_str = O.OracleParams(**dict(kw, useSplitStep=True, fluxScheme='kt'))
+    seq = O.march_step(field, O.build_operators(nX, nY, nT, p_seq, dz, np.float64), p_seq, 0.0, dz)
+    strang = O.march_step(field, O.build_operators(nX, nY, nT, p_str, dz, np.float64), p_str, 0.0, dz)
+    assert rel_l2(seq, field) > 1e-3                  # the step does something
+    assert rel_l2(seq, strang) < 1e-12
+    # and the full-step operators are not interchangeable with the half ones
+    p_half = O.OracleParams(**dict(kw, useSplitStep=True))
+    ops = O.build_operators(nX, nY, nT, p_half, dz, np.float64)
+    once = O.attenuation_step(O.angular_spectrum_step(field, ops['HH_half'], ops['abl_half']),
+                              ops['afilt_half'])
+    assert rel_l2(seq, once) > 1e-3
