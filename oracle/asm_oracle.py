@@ -20,12 +20,16 @@ reference's jitted kernels use ``jax.numpy.fft`` (XLA/ducc); both are exact DFTs
 up to rounding, and both c2r transforms discard the imaginary part of the DC and
 Nyquist bins, which the pipeline relies on (checked in the tests).
 
-Each function is pinned in ``tests/test_oracle_pins.py`` by checks that do not
-retype its formula: closed forms, invariants, the reference's known-answer test and
-published convergence orders.  Pinned only by restatement: the scheme dispatch
-(``nonlinear_kind``, ``beta_tvd_for``, ASS:1454-1492) and the place of the nonlinear
-update inside the sequential step (ASS:874-880); the linear part of every composite
-step is pinned.
+Each function is pinned in ``tests/test_oracle_pins.py`` and
+``tests/test_oracle_boundaries.py`` by checks that do not retype its formula: closed
+forms, invariants, the reference's known-answer test, published convergence orders
+and the published boundary/centre intensity ratios (validate_boundaries.py test 2).
+The super-absorbing step is pinned on plane waves (the outgoing wave is removed, the
+y pass acts on the x-corrected spectrum); the scheme dispatch (``nonlinear_kind``,
+ASS:1454-1492) by the flux scale and z order of the scheme it selects; the sequential
+step's operator order (ASS:874-880) by low-pass operators.  Pinned only by
+restatement: ``beta_tvd_for`` (ASS:1465-1466), whose value the TVD limiter cannot show
+for the driver's beta in [1, 2] (min(beta, 1) = 1).
 """
 from __future__ import annotations
 
