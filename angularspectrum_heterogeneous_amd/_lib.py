@@ -16,7 +16,9 @@ LIB_PATH = os.path.join(HERE, "libaship.so")
 ASH_OK = 0
 ASH_E_ARG, ASH_E_UNSUPPORTED, ASH_E_HIP, ASH_E_STATE, ASH_E_HOST = -1, -2, -3, -4, -5
 NL_NONE, NL_KT, NL_RUSANOV, NL_TVD, NL_KT_MINMOD = 0, 1, 2, 3, 4
-SUBCYCLE_OFF, SUBCYCLE_COLUMN = 0, 1
+SUBCYCLE_OFF, SUBCYCLE_COLUMN, SUBCYCLE_GLOBAL = 0, 1, 2
+TOF_OFF, TOF_ENVELOPE, TOF_MATCHED = 0, 1, 2
+MAX_SUBSTEPS = 1024
 PROFILE_QUADRATIC, PROFILE_WENDLAND = 0, 1
 
 c_i32 = ctypes.c_int32
@@ -54,6 +56,7 @@ class AshStepIO(ctypes.Structure):
         ("src_dense_dev", c_vp * 4),
         ("src_mask_dev", c_vp), ("src_delay_dev", c_vp), ("taxis_dev", c_vp),
         ("src_f0", c_dbl), ("src_p0", c_dbl), ("src_ncycles", c_dbl), ("src_dur", c_dbl),
+        ("tof_dev", c_vp),
     ]
 
 
@@ -61,6 +64,7 @@ class AshStepIO(ctypes.Structure):
 _PROTOS = {
     "ash_abi_version": ([], c_i32),
     "ash_last_error": ([], ctypes.c_char_p),
+    "ash_centred_wavenumbers": ([c_i32, c_dbl, ctypes.POINTER(c_dbl)], c_i32),
     "ash_plan_create": ([ctypes.POINTER(AshDesc), ctypes.POINTER(c_vp)], c_i32),
     "ash_plan_destroy": ([c_vp], None),
     "ash_plan_set_dz": ([c_vp, c_dbl], c_i32),
@@ -72,6 +76,7 @@ _PROTOS = {
     "ash_angular_spectrum_step": ([c_vp, c_vp, c_vp, c_i32, c_vp], c_i32),
     "ash_nonlinear_step": ([c_vp, c_vp, c_vp, c_i32, c_dbl, c_dbl, c_i32, c_vp], c_i32),
     "ash_max_abs": ([c_vp, c_vp, c_vp, c_vp], c_i32),
+    "ash_plan_subcycle_capped": ([c_vp, ctypes.POINTER(ctypes.c_int64)], c_i32),
     "ash_freq_weighted_boundary_step": ([c_vp, c_vp, c_vp, c_vp], c_i32),
     "ash_super_absorbing_step": ([c_vp, c_vp, c_vp, c_vp], c_i32),
     "ash_apply_phase_screen": ([c_vp, c_vp, c_vp, c_vp, c_vp, c_vp], c_i32),
@@ -84,6 +89,7 @@ _PROTOS = {
     "ash_batch_create": ([ctypes.POINTER(c_vp)], c_i32),
     "ash_batch_destroy": ([c_vp], None),
     "ash_batch_step": ([c_vp, c_vp, c_vp, c_i32, c_vp], c_i32),
+    "ash_batch_set_branches": ([c_vp, c_i32], c_i32),
     "ash_net_param_count": ([], ctypes.c_int64),
     "ash_net_create": ([c_i32, c_i32, ctypes.POINTER(c_vp)], c_i32),
     "ash_net_destroy": ([c_vp], None),
@@ -95,6 +101,7 @@ _PROTOS = {
                      c_i32),
     "ash_net_apply": ([c_vp, c_vp, c_i32, c_dbl, c_dbl, c_dbl, c_vp, c_vp], c_i32),
     "ash_plan_set_learned": ([c_vp, c_vp, c_dbl], c_i32),
+    "ash_plan_set_tof": ([c_vp, c_i32, c_dbl, c_dbl, c_dbl, c_vp, c_dbl, c_vp], c_i32),
 }
 
 KERNEL_KINDS = ("xfft_fwd", "ypass", "xfft_inv", "ysa", "column_nl", "column_taper", "column_final",
@@ -103,7 +110,7 @@ KERNEL_KINDS = ("xfft_fwd", "ypass", "xfft_inv", "ysa", "column_nl", "column_tap
 EXPORTED = tuple(_PROTOS)
 
 # ASH_ABI_VERSION of include/aship.h that the prototypes and structures above follow
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 _lib: Optional[ctypes.CDLL] = None
 

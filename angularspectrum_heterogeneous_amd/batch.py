@@ -4,7 +4,8 @@ The reference solves one configuration per ``angular_spectrum_solve`` call
 (ASS:1217); this module runs a list of (initial_field, SolverParams) instances
 of uneven grid sizes side by side.  Every z-step of the batch is one
 ``ash_batch_step`` call: the plain steps of all active instances are captured
-into one hipGraph and replayed with a single graph launch.  Each instance
+into one hipGraph, spread over up to four concurrent branches, and replayed
+with a single graph launch.  Each instance
 keeps the single-solve semantics of ``angular_spectrum_solve`` (its own dZ,
 stability restart, partial last step, outputs), so its results equal a
 single-instance run.
@@ -56,15 +57,17 @@ class BatchSolve:
     ``run()`` returns, per instance, the tuple of ``angular_spectrum_solve``:
     (field, pnp, ppp, pI, pIloss, zaxis, pax).  With ``autotune`` every plan
     picks its launch configuration by timing (``MarchPlan.autotune``); the
-    choice does not change the arithmetic.
+    choice does not change the arithmetic.  ``branches`` (1-4) is the number of
+    concurrent branches the instances are spread over in each step's graph.
     """
 
     def __init__(self, instances: Sequence[Tuple[object, SolverParams]], check_every: int = 32,
-                 autotune: bool = False, release_inputs: bool = True):
+                 autotune: bool = False, release_inputs: bool = True, branches: int = 4):
         self.lib = _lib.load()
         h = ctypes.c_void_p()
         check(self.lib.ash_batch_create(ctypes.byref(h)), "ash_batch_create")
         self.h = h
+        check(self.lib.ash_batch_set_branches(h, int(branches)), "ash_batch_set_branches")
         self.items: List[_Instance] = [_Instance(f, p, check_every, autotune) for f, p in instances]
         self.release_inputs = release_inputs
         self.steps_launched = 0

@@ -9,12 +9,15 @@
 #include <cstring>
 #include <exception>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/aship.h"
 #include "kernels.h"
+#include "generic.h"
 #include "learned.h"
 
 using namespace ash;
@@ -58,8 +61,11 @@ int caught_exception() noexcept {
     if (e_ != hipSuccess) return fail(ASH_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// Sizes of the FFT kernels; every other size runs the general-size kernels (generic.h).
 bool line_supported(int n) { return n == 64 || n == 128 || n == 192 || n == 256 || n == 384; }
 bool col_supported(int m) { return m == 128 || m == 256 || m == 512 || m == 1024; }
+constexpr int GEN_MAX_LINE = GEN_MAXL;  // general-size line kernels: 2 .. 1024
+constexpr int GEN_MAX_NT = 2048;         // general-length column kernel: 4 .. 2048
 
 // Python's round(): half to even
 long py_round(double v) {
@@ -85,15 +91,40 @@ std::vector<double> taper(int N, int n, int profile) {
   return v;
 }
 
+// Sum in NumPy's order (its pairwise summation of a contiguous float64 array):
+// below 8 terms a plain loop; up to 128 eight interleaved partial sums combined as
+// ((r0+r1)+(r2+r3))+((r4+r5)+(r6+r7)), then the remainder; above, the halves
+// (split at a multiple of 8) recursively.  The reference builds its wavenumber
+// grids with np.mean (ASS:404-408); a sum in another order can move a grid value
+// by an ulp, and on grids where some mode sits exactly at the evanescent cutoff
+// (k_z = 0, where the obliquity factor of the attenuation jumps from 0 to 1) that
+// ulp decides the mode.
+double numpy_sum(const double* a, long n) {
+  if (n < 8) {
+    double r = 0.0;
+    for (long i = 0; i < n; ++i) r += a[i];
+    return r;
+  }
+  if (n <= 128) {
+    double r[8];
+    for (int j = 0; j < 8; ++j) r[j] = a[j];
+    long i = 8;
+    for (; i < n - (n % 8); i += 8)
+      for (int j = 0; j < 8; ++j) r[j] += a[i + j];
+    double res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+    for (; i < n; ++i) res += a[i];
+    return res;
+  }
+  long n2 = n / 2;
+  n2 -= n2 % 8;
+  return numpy_sum(a, n2) + numpy_sum(a + n2, n - n2);
+}
+
 // (n-1)-normalised mean-subtracted grid (ASS:403-408), natural FFT order.
 std::vector<double> centred_natural(int n, double d, double extra_div) {
   std::vector<double> k(n);
-  double mean = 0.0;
-  for (int j = 0; j < n; ++j) {
-    k[j] = (double)j / (n - 1) / d * 2 * M_PI / extra_div;
-    mean += k[j];
-  }
-  mean /= n;
+  for (int j = 0; j < n; ++j) k[j] = (double)j / (n - 1) / d * 2 * M_PI / extra_div;
+  const double mean = numpy_sum(k.data(), n) / n;
   for (int j = 0; j < n; ++j) k[j] -= mean;
   std::vector<double> nat(n);
   for (int i = 0; i < n; ++i) nat[i] = k[(i + n / 2) % n];
@@ -113,11 +144,13 @@ struct ash_plan {
   double* ibefore = nullptr;
   int* d_row = nullptr;
   double* obl_acc = nullptr;  // [2] power-weighted obliquity sums of the current step
+  unsigned* red = nullptr;    // [RED_WORDS] max|p| partials, global max, sub-cycling cap counter
   Tables T{};
   std::vector<void*> allocs;
   int64_t ws_bytes = 0;
   bool loaded = false;
   int host_row = -1;
+  bool gen_x = false, gen_y = false, gen_t = false;  // axis runs the general-size kernels
   int sa_band = 0;      // last row index (from each edge) with a nonzero super-absorbing y weight
   int precise_ops = 0;  // evaluate the k-space operators in FP64 (set from ASH_PRECISE_OPS)
   // graph of a plain step
@@ -139,6 +172,10 @@ struct ash_plan {
   double dpdt_max = 0.0;     // KT where max|dp/dt| of a column exceeds this (Pa/s)
   float* rscratch = nullptr;  // real field between the column passes [nX*nY*nT]
   int32_t* nflag = nullptr;   // 1 where a column took the KT update [nX*nY]
+  // time of flight (ash_plan_set_tof)
+  int tof_mode = 0;
+  double tof_ratio = 1.0, tof_t0 = 0.0, tof_dt = 0.0, tof_tref = 0.0;
+  float2* tofref = nullptr;   // rfft of the matched-filter reference trace [MP]
 };
 
 struct ash_net {
@@ -163,9 +200,21 @@ struct ash_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   hipStream_t cap = nullptr;
+  // concurrent branches of the captured step: instance i runs on branch[lane[i]]
+  int nbranch = 4;
+  std::vector<hipStream_t> branch;
+  std::vector<hipEvent_t> fork_join;  // [0] fork, [1 + k] join of branch k
 };
 
 namespace {
+
+// Layout of ash_plan::red (and of the no-allocation scratch, same kernels):
+// per-block max|p| partials, then the global maximum, then the count of columns
+// (or fields) whose sub-cycling hit MAX_SUBSTEPS.
+constexpr int RED_PARTIALS = 65536;  // >= any column grid (<= 65535) and nX nY / 64 (<= 2304)
+constexpr int RED_GMAX = RED_PARTIALS;
+constexpr int RED_SUBCAP = RED_PARTIALS + 1;
+constexpr int RED_WORDS = RED_PARTIALS + 64;
 
 template <class T>
 int upload(ash_plan* p, const std::vector<T>& h, const T** out) {
@@ -184,6 +233,12 @@ std::vector<float2> twiddles(int L) {
   for (int k = 0; k < L; ++k) {
     double a = -2.0 * M_PI * (double)k / (double)L;
     t[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+    // exact values at the multiples of a quarter turn
+    if ((4L * k) % L == 0) {
+      const int q = (int)((4L * k) / L);
+      const float c[4] = {1.f, 0.f, -1.f, 0.f}, sn[4] = {0.f, -1.f, 0.f, 1.f};
+      t[k] = make_float2(c[q], sn[q]);
+    }
   }
   return t;
 }
@@ -198,17 +253,17 @@ int build_tables(ash_plan* p) {
   // ASM t-wavenumber of the kept (positive) bins: fftshifted index (m + nT/2) % nT
   std::vector<double> ktc(nT);
   {
-    double mean = 0.0;
-    for (int j = 0; j < nT; ++j) { ktc[j] = (double)j / (nT - 1) / d.dT * 2 * M_PI / d.c0; mean += ktc[j]; }
-    mean /= nT;
-    double sabs = 0.0;
-    int cnt = 0;
+    for (int j = 0; j < nT; ++j) ktc[j] = (double)j / (nT - 1) / d.dT * 2 * M_PI / d.c0;
+    const double mean = numpy_sum(ktc.data(), nT) / nT;
+    std::vector<double> nzabs;
     for (int j = 0; j < nT; ++j) {
       ktc[j] -= mean;
-      if (ktc[j] != 0.0) { sabs += std::fabs(ktc[j]); ++cnt; }
+      if (ktc[j] != 0.0) nzabs.push_back(std::fabs(ktc[j]));
     }
+    const int cnt = (int)nzabs.size();
     p->kt_nonzero = cnt;
-    p->lambda_char = cnt ? d.c0 / (sabs / cnt) : d.c0 / (1.0 / d.dT);
+    // lambda_char = c0 / mean|kt| over the nonzero bins (ASS:415)
+    p->lambda_char = cnt ? d.c0 / (numpy_sum(nzabs.data(), cnt) / cnt) : d.c0 / (1.0 / d.dT);
   }
   std::vector<double> kt(MP, 0.0), ka(MP, 0.0), al(MP, 0.0), as(MP, 0.0);
   const int kept_hi = (nT + 1) / 2 - 1;  // natural bins 1..ceil(nT/2)-1 are kept
@@ -306,8 +361,10 @@ int build_tables(ash_plan* p) {
   }
   if ((rc = upload(p, twiddles(nX), &p->T.twx))) return rc;
   if ((rc = upload(p, twiddles(nY), &p->T.twy))) return rc;
-  if ((rc = upload(p, twiddles(M), &p->T.twm))) return rc;
-  {
+  p->T.twgx = p->T.twx;  // the same full-length tables serve the general-size kernels
+  p->T.twgy = p->T.twy;
+  if ((rc = upload(p, twiddles(nT), &p->T.twgt))) return rc;
+  if (!p->gen_t) {
     // pass-major twiddles of the column transform, [r * Ns + k] = W_{Ns R}^{k r}
     auto radices = [&](bool inv) {
       std::vector<int> r;
@@ -335,14 +392,6 @@ int build_tables(ash_plan* p) {
       }
     }
     if ((rc = upload(p, tc, &p->T.twc))) return rc;
-  }
-  {
-    std::vector<float2> tn(M + 1);
-    for (int k = 0; k <= M; ++k) {
-      double a = -2.0 * M_PI * (double)k / (double)nT;
-      tn[k] = make_float2((float)std::cos(a), (float)std::sin(a));
-    }
-    if ((rc = upload(p, tn, &p->T.twn))) return rc;
   }
   return ASH_OK;
 }
@@ -470,6 +519,8 @@ void launch_col_M(ash_plan* p, const ColDesc& cd, hipStream_t s) {
     case COL_STORE: launch_col_MN<M, COL_STORE, NL_NONE>(p, cd, s); break;
     case COL_RAWNL: launch_col_MM<M, COL_RAWNL>(p, cd, s); break;
     case COL_TOREAL: launch_col_MN<M, COL_TOREAL, NL_NONE>(p, cd, s); break;
+    case COL_MAXABS: launch_col_MN<M, COL_MAXABS, NL_NONE>(p, cd, s); break;
+    case COL_TOF: launch_col_MN<M, COL_TOF, NL_NONE>(p, cd, s); break;
     default: break;
   }
 }
@@ -484,16 +535,33 @@ void launch_col_M(ash_plan* p, const ColDesc& cd, hipStream_t s) {
     default: break;                            \
   }
 
+dim3 gen_grid(const ash_plan* p, int n_line, int n_other) { return dim3(n_other * (p->g.MP / gen_lines(n_line))); }
+
 void xfwd(ash_plan* p, hipStream_t s) {
   ProfScope ps(p, ASH_K_XFWD, s);
+  if (p->gen_x) {
+    hipLaunchKernelGGL(k_gx_fwd, gen_grid(p, p->g.nX, p->g.nY), dim3(GNT), 0, s, p->spec, p->T, p->g);
+    return;
+  }
   DISPATCH_L(p->g.nX, launch_xfwd_L<L>(p, s));
 }
 void ypass(ash_plan* p, const OpDesc& op, hipStream_t s) {
   ProfScope ps(p, ASH_K_YPASS, s);
+  if (p->gen_y) {
+    hipLaunchKernelGGL(k_gy_pass, gen_grid(p, p->g.nY, p->g.nX), dim3(GNT), 0, s, p->spec, p->T, p->g, op);
+    return;
+  }
   DISPATCH_L(p->g.nY, launch_ypass_L<L>(p, op, s));
 }
 void xinv(ash_plan* p, int epi, const EpiDesc& e, hipStream_t s) {
   ProfScope ps(p, ASH_K_XINV, s);
+  if (p->gen_x) {
+    const dim3 grid = gen_grid(p, p->g.nX, p->g.nY);
+    if (epi == 0) hipLaunchKernelGGL(k_gx_inv<0>, grid, dim3(GNT), 0, s, p->spec, p->T, p->g, e);
+    else if (epi == 1) hipLaunchKernelGGL(k_gx_inv<1>, grid, dim3(GNT), 0, s, p->spec, p->T, p->g, e);
+    else hipLaunchKernelGGL(k_gx_inv<2>, grid, dim3(GNT), 0, s, p->spec, p->T, p->g, e);
+    return;
+  }
   if (epi == 0) { DISPATCH_L(p->g.nX, (launch_xinv_L<L, 0>(p, e, s))); }
   else if (epi == 1) { DISPATCH_L(p->g.nX, (launch_xinv_L<L, 1>(p, e, s))); }
   else { DISPATCH_L(p->g.nX, (launch_xinv_L<L, 2>(p, e, s))); }
@@ -506,16 +574,72 @@ void launch_ypower_L(ash_plan* p, hipStream_t s) {
 }
 void ypower(ash_plan* p, hipStream_t s) {
   ProfScope ps(p, ASH_K_YPASS, s);
+  if (p->gen_y) {
+    hipLaunchKernelGGL(k_gy_power, gen_grid(p, p->g.nY, p->g.nX), dim3(GNT), 0, s, p->spec, p->T, p->g, p->obl_acc);
+    return;
+  }
   DISPATCH_L(p->g.nY, launch_ypower_L<L>(p, s));
 }
 void ysa(ash_plan* p, const ScreenDesc& sd, hipStream_t s) {
   ProfScope ps(p, ASH_K_YSA, s);
+  if (p->gen_y) {
+    hipLaunchKernelGGL(k_gy_sa, gen_grid(p, p->g.nY, p->g.nX), dim3(GNT), 0, s, p->spec, p->T, p->g, sd);
+    return;
+  }
   DISPATCH_L(p->g.nY, launch_ysa_L<L>(p, sd, s));
 }
+
+// General-length column kernel: S real samples per lane, 64 S >= nT.
+int gen_col_samples(int nT) { return nT <= 128 ? 2 : nT <= 256 ? 4 : nT <= 512 ? 8 : nT <= 1024 ? 16 : 32; }
+template <int S, int MODE, int NLK>
+void launch_gcol_SMN(ash_plan* p, const ColDesc& cd, hipStream_t s) {
+  const int ncols = p->g.nX * p->g.nY;
+  const int blocks = std::min((ncols + GC_WPB - 1) / GC_WPB, p->col_grid);
+  hipLaunchKernelGGL((k_gcolumn<S, MODE, NLK>), dim3(blocks), dim3(256), 0, s, p->spec, p->T, p->g, cd);
+}
+template <int S, int MODE>
+void launch_gcol_SM(ash_plan* p, const ColDesc& cd, hipStream_t s) {
+  switch (cd.nl) {
+    case NL_KT: launch_gcol_SMN<S, MODE, NL_KT>(p, cd, s); break;
+    case NL_KT_MINMOD: launch_gcol_SMN<S, MODE, NL_KT_MINMOD>(p, cd, s); break;
+    case NL_RUSANOV: launch_gcol_SMN<S, MODE, NL_RUSANOV>(p, cd, s); break;
+    case NL_TVD: launch_gcol_SMN<S, MODE, NL_TVD>(p, cd, s); break;
+    default: launch_gcol_SMN<S, MODE, NL_NONE>(p, cd, s); break;
+  }
+}
+template <int S>
+void launch_gcol_S(ash_plan* p, const ColDesc& cd, hipStream_t s) {
+  switch (cd.mode) {
+    case COL_LOAD: launch_gcol_SMN<S, COL_LOAD, NL_NONE>(p, cd, s); break;
+    case COL_NL: launch_gcol_SM<S, COL_NL>(p, cd, s); break;
+    case COL_TAPER: launch_gcol_SMN<S, COL_TAPER, NL_NONE>(p, cd, s); break;
+    case COL_FINAL:
+      if (cd.n_src > 0 || cd.smask) launch_gcol_SMN<S, COL_FINAL_SRC, NL_NONE>(p, cd, s);
+      else launch_gcol_SMN<S, COL_FINAL, NL_NONE>(p, cd, s);
+      break;
+    case COL_STORE: launch_gcol_SMN<S, COL_STORE, NL_NONE>(p, cd, s); break;
+    case COL_RAWNL: launch_gcol_SM<S, COL_RAWNL>(p, cd, s); break;
+    case COL_TOREAL: launch_gcol_SMN<S, COL_TOREAL, NL_NONE>(p, cd, s); break;
+    case COL_MAXABS: launch_gcol_SMN<S, COL_MAXABS, NL_NONE>(p, cd, s); break;
+    case COL_TOF: launch_gcol_SMN<S, COL_TOF, NL_NONE>(p, cd, s); break;
+    default: break;
+  }
+}
+
 void column(ash_plan* p, const ColDesc& cd, hipStream_t s) {
   const int kind = cd.mode == COL_NL ? ASH_K_COL_NL : cd.mode == COL_TAPER ? ASH_K_COL_TAPER
                  : cd.mode == COL_FINAL ? ASH_K_COL_FINAL : ASH_K_COL_OTHER;
   ProfScope ps(p, kind, s);
+  if (p->gen_t) {
+    switch (gen_col_samples(p->g.nT)) {
+      case 2: launch_gcol_S<2>(p, cd, s); break;
+      case 4: launch_gcol_S<4>(p, cd, s); break;
+      case 8: launch_gcol_S<8>(p, cd, s); break;
+      case 16: launch_gcol_S<16>(p, cd, s); break;
+      default: launch_gcol_S<32>(p, cd, s); break;
+    }
+    return;
+  }
   switch (p->g.M) {
     case 128: launch_col_M<128>(p, cd, s); break;
     case 256: launch_col_M<256>(p, cd, s); break;
@@ -532,6 +656,8 @@ ColDesc col_base(const ash_plan* p) {
   c.dz = p->dz;
   c.cfl_target = (float)p->d.cflTarget;
   c.subcycle = 0;
+  c.gmax = p->red + RED_GMAX;
+  c.subcap = p->red + RED_SUBCAP;
   c.cax = (p->d.nX / 2) * p->d.nY + p->d.nY / 2;
   return c;
 }
@@ -540,6 +666,30 @@ int nl_kind(const ash_plan* p) {
   if (!p->d.useSplitStep) return NL_RUSANOV;
   if (p->d.fluxScheme == 1) return NL_KT;
   return p->d.useTVD ? NL_TVD : NL_RUSANOV;
+}
+
+bool is_kt(int nl) { return nl == NL_KT || nl == NL_KT_MINMOD; }
+
+int col_blocks(const ash_plan* p, int M) {
+  if (p->gen_t) return std::min((p->g.nX * p->g.nY + GC_WPB - 1) / GC_WPB, p->col_grid);
+  const int per_block = M == 128 ? ColCfg<128>::WPB * ColCfg<128>::CPW
+                      : M == 256 ? ColCfg<256>::WPB * ColCfg<256>::CPW
+                      : M == 512 ? ColCfg<512>::WPB * ColCfg<512>::CPW
+                                 : ColCfg<1024>::WPB * ColCfg<1024>::CPW;
+  const int ncols = p->g.nX * p->g.nY;
+  return std::min((ncols + per_block - 1) / per_block, p->col_grid);
+}
+
+// Global max|p| of the field the nonlinear column pass `cd` will see (same
+// transform and taper), into red[RED_GMAX]: a read-only column pass writing one
+// partial per block, then a one-block reduction.
+void global_max_of_nl_input(ash_plan* p, const ColDesc& cd, hipStream_t s) {
+  ColDesc probe = cd;
+  probe.mode = COL_MAXABS;
+  probe.nl = NL_NONE;
+  probe.partial = p->red;
+  column(p, probe, s);
+  hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, s, p->red, col_blocks(p, p->g.M), p->red + RED_GMAX);
 }
 
 double beta_tvd(const ash_plan* p, double dz) {
@@ -624,7 +774,7 @@ void enqueue_composite(ash_plan* p, double dz, int epi_last, const EpiDesc& elas
   cd.dz = dz;
   cd.beta_tvd = (float)beta_tvd(p, dz);
   cd.nl = nl_kind(p);
-  cd.subcycle = p->d.subcycle;
+  cd.subcycle = is_kt(cd.nl) ? p->d.subcycle : 0;  // only the KT update sub-cycles (ASS:842-859)
   if (p->d.useSplitStep) {
     // attenuation half + propagator half, merged (both diagonal in k-space)
     xfwd(p, s);
@@ -636,6 +786,7 @@ void enqueue_composite(ash_plan* p, double dz, int epi_last, const EpiDesc& elas
     if (p->net) {
       learned_nl(p, cd, dz, s);
     } else {
+      if (cd.subcycle == SUBCYCLE_GLOBAL) global_max_of_nl_input(p, cd, s);
       cd.mode = COL_NL;
       column(p, cd, s);
     }
@@ -704,6 +855,21 @@ void enqueue_step(ash_plan* p, const ash_step_io* io, hipStream_t s) {
   cd.taxis = io->taxis_dev;
   cd.s_f0 = io->src_f0; cd.s_p0 = io->src_p0; cd.s_ncyc = io->src_ncycles; cd.s_dur = io->src_dur;
   column(p, cd, s);
+  if (p->tof_mode != 0 && io->tof_dev) {
+    // time of flight of the field after the step (ASS:1538-1545), read only
+    ColDesc ct = col_base(p);
+    ct.mode = COL_TOF;
+    ct.row = p->d_row;
+    ct.max_rows = io->max_rows;
+    ct.tof = io->tof_dev;
+    ct.tof_mode = p->tof_mode;
+    ct.tof_ratio = (float)p->tof_ratio;
+    ct.tof_t0 = p->tof_t0;
+    ct.tof_dt = p->tof_dt;
+    ct.tof_tref = p->tof_tref;
+    ct.tofref = p->tofref;
+    column(p, ct, s);
+  }
   hipLaunchKernelGGL(k_advance_row, dim3(1), dim3(1), 0, s, p->d_row);
 }
 
@@ -739,6 +905,16 @@ int ash_abi_version(void) { return ASH_ABI_VERSION; }
 
 const char* ash_last_error(void) { return g_err.c_str(); }
 
+int ash_centred_wavenumbers(int32_t n, double d, double* out) try {
+  if (!out) return fail(ASH_E_ARG, "null argument");
+  if (n < 2 || !(d > 0)) return fail(ASH_E_ARG, "n must be >= 2 and d positive");
+  const std::vector<double> k = centred_natural(n, d, 1.0);
+  std::copy(k.begin(), k.end(), out);
+  return ASH_OK;
+} catch (...) {
+  return caught_exception();
+}
+
 int ash_plan_create(const ash_desc* desc, ash_plan** out) try {
   if (!desc || !out) return fail(ASH_E_ARG, "null argument");
   *out = nullptr;
@@ -746,10 +922,13 @@ int ash_plan_create(const ash_desc* desc, ash_plan** out) try {
   if (d.nX <= 0 || d.nY <= 0 || d.nT <= 0) return fail(ASH_E_ARG, "grid sizes must be positive");
   if (!(d.dX > 0 && d.dY > 0 && d.dT > 0 && d.c0 > 0 && d.rho0 > 0))
     return fail(ASH_E_ARG, "dX, dY, dT, c0 and rho0 must be positive");
-  if (!line_supported(d.nX) || !line_supported(d.nY))
-    return fail(ASH_E_UNSUPPORTED, "nX and nY must be one of 64, 128, 192, 256, 384");
-  if (d.nT % 2 != 0 || !col_supported(d.nT / 2))
-    return fail(ASH_E_UNSUPPORTED, "nT must be one of 256, 512, 1024, 2048");
+  if (d.nX < 2 || d.nY < 2 || d.nX > GEN_MAX_LINE || d.nY > GEN_MAX_LINE)
+    return fail(ASH_E_UNSUPPORTED, "nX and nY must be between 2 and 1024");
+  if (d.nT < 4 || d.nT > GEN_MAX_NT) return fail(ASH_E_UNSUPPORTED, "nT must be between 4 and 2048");
+  if (d.subcycle < ASH_SUBCYCLE_OFF || d.subcycle > ASH_SUBCYCLE_GLOBAL)
+    return fail(ASH_E_ARG, "subcycle must be ASH_SUBCYCLE_OFF, _COLUMN or _GLOBAL");
+  if (d.subcycle != ASH_SUBCYCLE_OFF && !(d.useSplitStep && d.fluxScheme == 1))
+    return fail(ASH_E_ARG, "CFL sub-cycling applies to the split-step KT update only (fluxScheme kt)");
   // destroys the partial plan on every early return and on an exception
   std::unique_ptr<ash_plan, void (*)(ash_plan*)> guard(new ash_plan(), ash_plan_destroy);
   ash_plan* p = guard.get();
@@ -759,6 +938,11 @@ int ash_plan_create(const ash_desc* desc, ash_plan** out) try {
   p->g.M = d.nT / 2;
   p->g.MP = ((p->g.M + 1 + LF - 1) / LF) * LF;
   p->g.nfb = p->g.MP / LF;
+  p->g.nyq = d.nT % 2 == 0 ? p->g.M : -1;
+  p->gen_x = !line_supported(d.nX);
+  // the FFT y pass works on row pairs x, nX-1-x, whose |kx| agree only for even nX
+  p->gen_y = !line_supported(d.nY) || d.nX % 2 != 0;
+  p->gen_t = !(d.nT % 2 == 0 && col_supported(d.nT / 2));
   p->N = d.beta / (2 * d.c0 * d.c0 * d.c0 * d.rho0);
   if (const char* ev = std::getenv("ASH_PRECISE_OPS")) p->precise_ops = std::atoi(ev);
   p->dz = 0.0;
@@ -779,13 +963,16 @@ int ash_plan_create(const ash_desc* desc, ash_plan** out) try {
   if ((e = dalloc((void**)&p->spec, nspec * sizeof(float2))) != hipSuccess ||
       (e = dalloc((void**)&p->ibefore, (size_t)d.nX * d.nY * sizeof(double))) != hipSuccess ||
       (e = dalloc((void**)&p->d_row, 64)) != hipSuccess ||
-      (e = dalloc((void**)&p->obl_acc, 64)) != hipSuccess)
+      (e = dalloc((void**)&p->obl_acc, 64)) != hipSuccess ||
+      (e = dalloc((void**)&p->red, RED_WORDS * sizeof(unsigned))) != hipSuccess)
     return fail(ASH_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
   // padding bins are read (masked) but never written: clear once
   if ((e = hipMemset(p->spec, 0, nspec * sizeof(float2))) != hipSuccess ||
-      (e = hipMemset(p->d_row, 0, 64)) != hipSuccess || (e = hipDeviceSynchronize()) != hipSuccess)
+      (e = hipMemset(p->d_row, 0, 64)) != hipSuccess ||
+      (e = hipMemset(p->red, 0, RED_WORDS * sizeof(unsigned))) != hipSuccess ||
+      (e = hipDeviceSynchronize()) != hipSuccess)
     return fail(ASH_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
-  p->ws_bytes += (int64_t)(nspec * sizeof(float2) + (size_t)d.nX * d.nY * sizeof(double));
+  p->ws_bytes += (int64_t)(nspec * sizeof(float2) + (size_t)d.nX * d.nY * sizeof(double) + RED_WORDS * sizeof(unsigned));
   *out = guard.release();
   return ASH_OK;
 } catch (...) {
@@ -815,6 +1002,7 @@ int ash_load_field(ash_plan* p, const float* field, uint32_t* stats, void* strea
   if (!p || !field) return fail(ASH_E_ARG, "null argument");
   hipStream_t s = (hipStream_t)stream;
   load_real(p, field, stats, p->ibefore, s);
+  hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, p->red + RED_SUBCAP, 0u);  // a new march
   p->loaded = true;
   return check_launch();
 } catch (...) {
@@ -854,7 +1042,7 @@ bool plain_step(const ash_plan* p, const ash_step_io* io) {
 bool same_io(const ash_step_io& a, const ash_step_io& b) {
   return a.pnp_dev == b.pnp_dev && a.ppp_dev == b.ppp_dev && a.pI_dev == b.pI_dev && a.pIloss_dev == b.pIloss_dev &&
          a.pax_dev == b.pax_dev && a.stats_dev == b.stats_dev && a.max_rows == b.max_rows &&
-         a.field_out_dev == b.field_out_dev && a.dz == b.dz;
+         a.field_out_dev == b.field_out_dev && a.dz == b.dz && a.tof_dev == b.tof_dev;
 }
 }  // namespace
 
@@ -894,71 +1082,85 @@ int ash_plan_autotune(ash_plan* p, int32_t* choices, void* stream) try {
   if (!p) return fail(ASH_E_ARG, "null plan");
   hipStream_t s = (hipStream_t)stream;
   const size_t nbytes = (size_t)p->g.nX * p->g.nY * p->g.MP * sizeof(float2);
-  float2* save = nullptr;
+  // the saved spectrum and the timing events are released on every exit path
+  std::unique_ptr<void, void (*)(void*)> save(nullptr, [](void* q) { if (q) (void)hipFree(q); });
+  auto ev_del = [](hipEvent_t e) { if (e) (void)hipEventDestroy(e); };
+  std::unique_ptr<std::remove_pointer<hipEvent_t>::type, decltype(ev_del)> e0(nullptr, ev_del), e1(nullptr, ev_del);
   if (p->loaded) {
-    HIP_TRY(hipMalloc(&save, nbytes));
-    HIP_TRY(hipMemcpyAsync(save, p->spec, nbytes, hipMemcpyDeviceToDevice, s));
+    void* q = nullptr;
+    HIP_TRY(hipMalloc(&q, nbytes));
+    save.reset(q);
+    HIP_TRY(hipMemcpyAsync(q, p->spec, nbytes, hipMemcpyDeviceToDevice, s));
   } else {
     HIP_TRY(hipMemsetAsync(p->spec, 0, nbytes, s));  // finite data for the trial sweeps
   }
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  HIP_TRY(hipEventCreate(&e0));
-  HIP_TRY(hipEventCreate(&e1));
+  {
+    hipEvent_t a = nullptr, b = nullptr;
+    HIP_TRY(hipEventCreate(&a));
+    e0.reset(a);
+    HIP_TRY(hipEventCreate(&b));
+    e1.reset(b);
+  }
   int rc = ASH_OK;
   auto timed = [&](auto&& launch) -> float {
     launch();  // warm-up
-    (void)hipEventRecord(e0, s);
+    (void)hipEventRecord(e0.get(), s);
     for (int r = 0; r < 5; ++r) launch();
-    (void)hipEventRecord(e1, s);
-    (void)hipEventSynchronize(e1);
+    (void)hipEventRecord(e1.get(), s);
+    (void)hipEventSynchronize(e1.get());
     float ms = 0.f;
-    if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = fail(ASH_E_HIP, "autotune timing failed");
+    if (hipEventElapsedTime(&ms, e0.get(), e1.get()) != hipSuccess) rc = fail(ASH_E_HIP, "autotune timing failed");
     return ms / 5.f;
   };
   const double dz = p->dz > 0 ? p->dz : p->d.dX;
-  // x-line transforms: forward + inverse per radix order
-  float best = 1e30f;
-  int pick = 0;
-  for (int v = 0; v < 2; ++v) {
-    p->rvx = v;
-    EpiDesc e = epi_base(p);
-    e.half = 1;
-    const float t = timed([&] { xfwd(p, s); xinv(p, 1, e, s); });
-    if (t < best) { best = t; pick = v; }
+  // x-line transforms: forward + inverse per radix order (the general-size
+  // kernels have no alternatives)
+  if (!p->gen_x) {
+    float best = 1e30f;
+    int pick = 0;
+    for (int v = 0; v < 2; ++v) {
+      p->rvx = v;
+      EpiDesc e = epi_base(p);
+      e.half = 1;
+      const float t = timed([&] { xfwd(p, s); xinv(p, 1, e, s); });
+      if (t < best) { best = t; pick = v; }
+    }
+    p->rvx = pick;
   }
-  p->rvx = pick;
   // y pass per radix order
-  best = 1e30f;
-  pick = 0;
-  const OpDesc op = make_op(p, true, dz / 2, true, dz * 0.5, dz);
-  for (int v = 0; v < 2; ++v) {
-    p->rvy = v;
-    const float t = timed([&] { ypass(p, op, s); });
-    if (t < best) { best = t; pick = v; }
+  if (!p->gen_y) {
+    float best = 1e30f;
+    int pick = 0;
+    const OpDesc op = make_op(p, true, dz / 2, true, dz * 0.5, dz);
+    for (int v = 0; v < 2; ++v) {
+      p->rvy = v;
+      const float t = timed([&] { ypass(p, op, s); });
+      if (t < best) { best = t; pick = v; }
+    }
+    p->rvy = pick;
   }
-  p->rvy = pick;
   // persistent grid of the column kernels (timed on the nonlinear pass)
-  ColDesc cd = col_base(p);
-  cd.dz = dz;
-  cd.beta_tvd = (float)beta_tvd(p, dz);
-  cd.nl = nl_kind(p);
-  cd.subcycle = 0;
-  cd.taper = 1;
-  cd.mode = COL_NL;
-  best = 1e30f;
-  pick = 2048;
-  for (int g : {512, 1024, 2048, 4096}) {
-    p->col_grid = g;
-    const float t = timed([&] { column(p, cd, s); });
-    if (t < best) { best = t; pick = g; }
+  {
+    ColDesc cd = col_base(p);
+    cd.dz = dz;
+    cd.beta_tvd = (float)beta_tvd(p, dz);
+    cd.nl = nl_kind(p);
+    cd.subcycle = 0;
+    cd.taper = 1;
+    cd.mode = COL_NL;
+    float best = 1e30f;
+    int pick = 2048;
+    for (int g : {512, 1024, 2048, 4096}) {
+      p->col_grid = g;
+      const float t = timed([&] { column(p, cd, s); });
+      if (t < best) { best = t; pick = g; }
+    }
+    p->col_grid = pick;
   }
-  p->col_grid = pick;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   if (save) {
-    HIP_TRY(hipMemcpyAsync(p->spec, save, nbytes, hipMemcpyDeviceToDevice, s));
+    // restore the loaded field (also after a failed timing)
+    HIP_TRY(hipMemcpyAsync(p->spec, save.get(), nbytes, hipMemcpyDeviceToDevice, s));
     HIP_TRY(hipStreamSynchronize(s));
-    (void)hipFree(save);
   }
   ++p->epoch;
   if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
@@ -999,7 +1201,22 @@ void ash_batch_destroy(ash_batch* b) {
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   if (b->cap) (void)hipStreamDestroy(b->cap);
+  for (hipStream_t st : b->branch) (void)hipStreamDestroy(st);
+  for (hipEvent_t e : b->fork_join) (void)hipEventDestroy(e);
   delete b;
+}
+
+int ash_batch_set_branches(ash_batch* b, int32_t n) try {
+  if (!b) return fail(ASH_E_ARG, "null batch");
+  if (n < 1 || n > ASH_BATCH_MAX_BRANCHES) return fail(ASH_E_ARG, "branches must be 1..4");
+  if (n != b->nbranch) {
+    b->nbranch = n;
+    if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+    if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+  }
+  return ASH_OK;
+} catch (...) {
+  return caught_exception();
 }
 
 int ash_batch_step(ash_batch* b, ash_plan* const* plans, const ash_step_io* ios, int32_t n, void* stream) try {
@@ -1024,10 +1241,49 @@ int ash_batch_step(ash_batch* b, ash_plan* const* plans, const ash_step_io* ios,
       if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
       if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
       if (!b->cap) HIP_TRY(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
+      // instances go to nbranch concurrent branches of the graph (fork/join on
+      // events), balanced by their voxel counts (largest first)
+      const int nb = std::min(b->nbranch, n);
+      while ((int)b->branch.size() < nb) {
+        hipStream_t st = nullptr;
+        HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));  // default priority
+        b->branch.push_back(st);
+      }
+      while ((int)b->fork_join.size() < nb + 1) {
+        hipEvent_t ev = nullptr;
+        HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        b->fork_join.push_back(ev);
+      }
+      std::vector<int> order(n), lane(n, 0);
+      std::vector<double> load(nb, 0.0);
+      for (int i = 0; i < n; ++i) order[i] = i;
+      auto work = [&](int i) { return (double)plans[i]->g.nX * plans[i]->g.nY * plans[i]->g.nT; };
+      std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return work(a) > work(c); });
+      for (int i : order) {
+        const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        lane[i] = k;
+        load[k] += work(i);
+      }
       HIP_TRY(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
-      for (int i = 0; i < n; ++i) enqueue_step(plans[i], &ios[i], b->cap);
+      hipError_t ce = hipSuccess;  // first error inside the capture (the capture is ended either way)
+      auto keep = [&ce](hipError_t e) { if (ce == hipSuccess) ce = e; };
+      if (nb > 1) {
+        keep(hipEventRecord(b->fork_join[0], b->cap));
+        for (int k = 0; k < nb; ++k) keep(hipStreamWaitEvent(b->branch[k], b->fork_join[0], 0));
+        for (int i = 0; i < n; ++i) enqueue_step(plans[i], &ios[i], b->branch[lane[i]]);
+        for (int k = 0; k < nb; ++k) {
+          keep(hipEventRecord(b->fork_join[1 + k], b->branch[k]));
+          keep(hipStreamWaitEvent(b->cap, b->fork_join[1 + k], 0));
+        }
+      } else {
+        for (int i = 0; i < n; ++i) enqueue_step(plans[i], &ios[i], b->cap);
+      }
       hipGraph_t g = nullptr;
       hipError_t e = hipStreamEndCapture(b->cap, &g);
+      if (ce != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(ASH_E_HIP, std::string("batch capture: ") + hipGetErrorString(ce));
+      }
       if (e != hipSuccess) return fail(ASH_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
       b->graph = g;
       HIP_TRY(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
@@ -1100,6 +1356,11 @@ int ash_nonlinear_step(ash_plan* p, const float* in, float* out, int32_t scheme,
                        int32_t subcycle, void* stream) try {
   if (!p || !in || !out) return fail(ASH_E_ARG, "null argument");
   if (scheme < ASH_NL_NONE || scheme > ASH_NL_KT_MINMOD) return fail(ASH_E_ARG, "unknown scheme");
+  if (subcycle < ASH_SUBCYCLE_OFF || subcycle > ASH_SUBCYCLE_GLOBAL) return fail(ASH_E_ARG, "unknown subcycle mode");
+  if (subcycle != ASH_SUBCYCLE_OFF && !is_kt(scheme))
+    return fail(ASH_E_ARG, "CFL sub-cycling applies to the KT schemes only");
+  hipStream_t s = (hipStream_t)stream;
+  const long ncols = (long)p->g.nX * p->g.nY;
   ColDesc cd = col_base(p);
   cd.mode = COL_RAWNL;
   cd.nl = scheme;
@@ -1108,7 +1369,12 @@ int ash_nonlinear_step(ash_plan* p, const float* in, float* out, int32_t scheme,
   cd.subcycle = subcycle;
   cd.rin = in;
   cd.rout = out;
-  column(p, cd, (hipStream_t)stream);
+  if (subcycle == ASH_SUBCYCLE_GLOBAL) {
+    const int nb = (int)((ncols + ABSMAX_COLS - 1) / ABSMAX_COLS);
+    hipLaunchKernelGGL(k_absmax_cols, dim3(nb), dim3(256), 0, s, in, ncols, p->g.nT, p->red);
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, s, p->red, nb, p->red + RED_GMAX);
+  }
+  column(p, cd, s);
   return check_launch();
 } catch (...) {
   return caught_exception();
@@ -1116,11 +1382,22 @@ int ash_nonlinear_step(ash_plan* p, const float* in, float* out, int32_t scheme,
 
 int ash_max_abs(ash_plan* p, const float* in, uint32_t* out_bits, void* stream) try {
   if (!p || !in || !out_bits) return fail(ASH_E_ARG, "null argument");
-  // the load kernel computes max|p| of its input; the spectrum it writes is
-  // the plan state, so this call also loads `in` as the current field
-  load_real(p, in, out_bits, nullptr, (hipStream_t)stream);
-  p->loaded = true;
+  hipStream_t s = (hipStream_t)stream;
+  const long ncols = (long)p->g.nX * p->g.nY;
+  const int nb = (int)((ncols + ABSMAX_COLS - 1) / ABSMAX_COLS);
+  hipLaunchKernelGGL(k_absmax_cols, dim3(nb), dim3(256), 0, s, in, ncols, p->g.nT, p->red);
+  hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, s, p->red, nb, out_bits);
   return check_launch();
+} catch (...) {
+  return caught_exception();
+}
+
+int ash_plan_subcycle_capped(ash_plan* p, int64_t* out) try {
+  if (!p || !out) return fail(ASH_E_ARG, "null argument");
+  unsigned v = 0;
+  HIP_TRY(hipMemcpy(&v, p->red + RED_SUBCAP, sizeof(v), hipMemcpyDeviceToHost));
+  *out = (int64_t)v;
+  return ASH_OK;
 } catch (...) {
   return caught_exception();
 }
@@ -1395,6 +1672,48 @@ int ash_net_apply(ash_net* n, float* u, int32_t ncols, double c, double dpdt_max
   return caught_exception();
 }
 
+int ash_plan_set_tof(ash_plan* p, int32_t mode, double env_ratio, double t0, double dt, const float* ref_dev,
+                     double t_ref_peak, void* stream) try {
+  if (!p) return fail(ASH_E_ARG, "null plan");
+  if (mode < ASH_TOF_OFF || mode > ASH_TOF_MATCHED) return fail(ASH_E_ARG, "unknown TOF mode");
+  if (mode != ASH_TOF_OFF && !(dt > 0)) return fail(ASH_E_ARG, "the time axis spacing must be positive");
+  if (mode == ASH_TOF_MATCHED && !ref_dev) return fail(ASH_E_ARG, "the matched filter needs a reference trace");
+  hipStream_t s = (hipStream_t)stream;
+  if (mode == ASH_TOF_MATCHED) {
+    if (!p->tofref) {
+      void* q = nullptr;
+      p->allocs.reserve(p->allocs.size() + 1);  // the push_back below cannot throw
+      HIP_TRY(hipMalloc(&q, (size_t)p->g.MP * sizeof(float2)));
+      p->allocs.push_back(q);
+      p->tofref = static_cast<float2*>(q);
+      p->ws_bytes += (int64_t)p->g.MP * sizeof(float2);
+    }
+    // rfft of the reference trace: the load pass of the column kernels on one column
+    const Geom gsave = p->g;
+    float2* ssave = p->spec;
+    p->g.nX = 1;
+    p->g.nY = 1;
+    p->spec = p->tofref;
+    ColDesc cd = col_base(p);
+    cd.mode = COL_LOAD;
+    cd.rin = ref_dev;
+    column(p, cd, s);
+    p->g = gsave;
+    p->spec = ssave;
+  }
+  p->tof_mode = mode;
+  p->tof_ratio = env_ratio;
+  p->tof_t0 = t0;
+  p->tof_dt = dt;
+  p->tof_tref = t_ref_peak;
+  ++p->epoch;  // the captured step graphs hold the previous TOF pass
+  if (p->gexec) { (void)hipGraphExecDestroy(p->gexec); p->gexec = nullptr; }
+  if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
+  return check_launch();
+} catch (...) {
+  return caught_exception();
+}
+
 int ash_plan_set_learned(ash_plan* p, ash_net* n, double dpdt_max) try {
   if (!p) return fail(ASH_E_ARG, "null plan");
   if (n) {
@@ -1402,6 +1721,8 @@ int ash_plan_set_learned(ash_plan* p, ash_net* n, double dpdt_max) try {
       return fail(ASH_E_UNSUPPORTED, "the learned step replaces the split-step KT update only");
     if (p->d.useNonlinearityObliquity)
       return fail(ASH_E_UNSUPPORTED, "the learned step does not take the beam obliquity scale");
+    if (p->d.subcycle != ASH_SUBCYCLE_OFF)
+      return fail(ASH_E_UNSUPPORTED, "the learned step replaces the un-sub-cycled KT update only");
     if (n->nT != p->d.nT) return fail(ASH_E_ARG, "network column length differs from the plan's nT");
     const size_t ncol = (size_t)p->d.nX * p->d.nY;
     // each buffer is allocated while missing, so a call after a failed one completes the set

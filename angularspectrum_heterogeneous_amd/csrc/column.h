@@ -12,8 +12,26 @@
 
 namespace ash {
 
-// Most KT sub-steps one column takes in one dZ with per-column CFL sub-cycling
+// Most KT sub-steps taken in one dZ by CFL sub-cycling.  A column (per-column
+// mode) or field (global mode) that would need more takes this many and is
+// counted in ColDesc::subcap, so the host can report it.
 constexpr int MAX_SUBSTEPS = 1024;
+
+// Sub-steps for a CFL number cfl at target `target` (_kt_flux_adaptive,
+// ASS:849-856): 1 at or below the target or for a field below 1e-30, else
+// ceil(cfl / target), bounded by MAX_SUBSTEPS.  *capped is set when the bound
+// (or a non-finite CFL number) took over.
+__device__ __forceinline__ int substeps_for(double cfl, double target, float umax, bool* capped) {
+  *capped = umax != umax;  // NaN amplitude: one step, reported
+  if (!(umax >= 1e-30f)) return 1;
+  const double q = cfl / target;
+  if (q <= 1.0) return 1;
+  if (!(q <= (double)MAX_SUBSTEPS)) {  // beyond the bound, or NaN
+    *capped = true;
+    return q > (double)MAX_SUBSTEPS ? MAX_SUBSTEPS : 1;
+  }
+  return (int)ceil(q);
+}
 
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -121,6 +139,302 @@ __device__ __forceinline__ void cfft(float2* v, float2* cb, const float2* twdir,
 // shuffle within the wave from lane src (caller masks column edges)
 __device__ __forceinline__ float shf(float x, int src) { return __shfl(x, src, 64); }
 
+// Lane view of one column: this lane's S real samples u[i] sit at t = t(i).
+// STRIDED: the FFT's ownership (pairs 2(ln + LPC e), +1); else contiguous
+// samples t0 .. t0 + S - 1.  BOUNDED: the column has nT <= LPC S samples and
+// the lane's samples at t >= nT are padding (zero, never read or written).
+template <int S, int LPC, bool STRIDED, bool BOUNDED>
+struct LaneCol {
+  int ln, lane, t0, nT;
+  __device__ __forceinline__ int t(int i) const {
+    if constexpr (STRIDED) return 2 * (ln + LPC * (i >> 1)) + (i & 1);
+    else return t0 + i;
+  }
+  __device__ __forceinline__ bool in(int i) const { return !BOUNDED || t(i) < nT; }
+  // store u[0..S) into the real column starting at dst
+  __device__ __forceinline__ void store(float* dst, const float* u) const {
+    static_assert(!(STRIDED && BOUNDED), "bounded columns use contiguous samples");
+    if constexpr (STRIDED) {
+      float2* d2 = reinterpret_cast<float2*>(dst);
+#pragma unroll
+      for (int e = 0; e < S / 2; ++e) d2[ln + LPC * e] = make_float2(u[2 * e], u[2 * e + 1]);
+    } else if constexpr (!BOUNDED) {
+      float4* d4 = reinterpret_cast<float4*>(dst + t0);
+#pragma unroll
+      for (int q = 0; q < S / 4; ++q) d4[q] = make_float4(u[4 * q], u[4 * q + 1], u[4 * q + 2], u[4 * q + 3]);
+    } else {
+#pragma unroll
+      for (int i = 0; i < S; ++i)
+        if (t0 + i < nT) dst[t0 + i] = u[i];
+    }
+  }
+};
+
+// Real-space part of a column pass (after the inverse transform, before the
+// forward one): taper, nonlinear update with its sub-cycling, source injection,
+// reductions and real outputs, on the S samples of this lane.  Shared by the
+// FFT column kernel and the general-length column kernel.
+template <int S, int LPC, bool STRIDED, bool BOUNDED, int MODE, int NLK>
+__device__ __forceinline__ void column_realspace(float (&u)[S], const LaneCol<S, LPC, STRIDED, BOUNDED>& L,
+                                                 const ColDesc& d, const Tables& T, int c, bool live, int ncols,
+                                                 int row, double Nn, int nsub_global, float& wmax) {
+  constexpr bool FIN = (MODE == COL_FINAL || MODE == COL_FINAL_SRC);
+  const float* tt = d.taper == 1 ? T.tapt_h : T.tapt_f;
+  if constexpr (MODE == COL_NL || MODE == COL_TAPER || MODE == COL_RAWNL) {
+    if (d.taper) {
+#pragma unroll
+      for (int i = 0; i < S; ++i) u[i] *= (L.in(i) ? tt[L.t(i)] : 0.f);
+    }
+    if constexpr (NLK != NL_NONE && MODE != COL_TAPER) {
+      int nsub = 1;
+      if (d.subcycle == SUBCYCLE_COLUMN) {
+        float mx = 0.f;
+#pragma unroll
+        for (int i = 0; i < S; ++i) mx = fmaxf(mx, fabsf(u[i]));
+        for (int o = LPC / 2; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+        bool capped;
+        nsub = substeps_for(Nn * (double)mx * d.dz / d.dT, (double)d.cfl_target, mx, &capped);
+        if (capped && live && L.ln == 0 && d.subcap) atomicAdd(d.subcap, 1u);
+      } else if (d.subcycle == SUBCYCLE_GLOBAL) {
+        nsub = nsub_global;
+      }
+      int nsub_w = nsub;
+      for (int o = 32; o >= LPC; o >>= 1) nsub_w = max(nsub_w, __shfl_xor(nsub_w, o, 64));
+      const float dz = (float)(d.dz / nsub);
+      const float coef = (float)(Nn / d.dT);
+      const int lsrc = L.lane - 1, rsrc = L.lane + 1;
+      const bool has_l = L.ln > 0, has_r = L.ln < LPC - 1;
+      for (int sub = 0; sub < nsub_w; ++sub) {
+        const bool act = sub < nsub;
+        if constexpr (NLK == NL_KT || NLK == NL_KT_MINMOD) {
+          float h[S + 4], r0[S], s1[S];
+          // halo: 2 samples from each neighbour lane (zero outside the column)
+          {
+            const float a = shf(u[S - 2], lsrc), b = shf(u[S - 1], lsrc);
+            const float cc = shf(u[0], rsrc), dd = shf(u[1], rsrc);
+            h[0] = has_l ? a : 0.f; h[1] = has_l ? b : 0.f;
+            h[S + 2] = has_r ? cc : 0.f; h[S + 3] = has_r ? dd : 0.f;
+#pragma unroll
+            for (int i = 0; i < S; ++i) h[i + 2] = u[i];
+          }
+          kt_rhs_local<S, NLK == NL_KT>(h, L.t0, L.nT, coef, r0);
+#pragma unroll
+          for (int i = 0; i < S; ++i) s1[i] = act ? u[i] + dz * r0[i] : u[i];
+          {
+            const float a = shf(s1[S - 2], lsrc), b = shf(s1[S - 1], lsrc);
+            const float cc = shf(s1[0], rsrc), dd = shf(s1[1], rsrc);
+            h[0] = has_l ? a : 0.f; h[1] = has_l ? b : 0.f;
+            h[S + 2] = has_r ? cc : 0.f; h[S + 3] = has_r ? dd : 0.f;
+#pragma unroll
+            for (int i = 0; i < S; ++i) h[i + 2] = s1[i];
+          }
+          kt_rhs_local<S, NLK == NL_KT>(h, L.t0, L.nT, coef, r0);
+          if (act) {
+#pragma unroll
+            for (int i = 0; i < S; ++i) u[i] = 0.5f * u[i] + 0.5f * (s1[i] + dz * r0[i]);
+          }
+        } else {
+          float h[S + 2];
+          {
+            const float a = shf(u[S - 1], lsrc), b = shf(u[0], rsrc);
+            h[0] = has_l ? a : 0.f;
+            h[S + 1] = has_r ? b : 0.f;
+#pragma unroll
+            for (int i = 0; i < S; ++i) h[i + 1] = u[i];
+          }
+          const float cf = (float)(Nn * (double)dz / d.dT);
+          float fl[S + 1];
+#pragma unroll
+          for (int i = 0; i < S + 1; ++i) {
+            const float uL = h[i], uR = h[i + 1];
+            const float a = fmaxf(fabsf(uL), fabsf(uR));
+            if constexpr (NLK == NL_RUSANOV) fl[i] = -(uL * uL + uR * uR) / 2.f - a * (uR - uL);
+            else fl[i] = 0.5f * (-0.5f * (uL * uL + uR * uR)) - 0.5f * a * (uR - uL);
+          }
+#pragma unroll
+          for (int i = 0; i < S; ++i) {
+            const int t = L.t0 + i;
+            if (act && t >= 1 && t <= L.nT - 2) {
+              const float fd = fl[i + 1] - fl[i];
+              if constexpr (NLK == NL_RUSANOV) {
+                u[i] += -cf * fd;
+              } else {
+                const float dm = h[i + 1] - h[i], dp = h[i + 2] - h[i + 1];
+                const float eps = 1e-30f, b = d.beta_tvd;
+                const float r = dp / (dm + eps * sgnf(dm + eps));
+                const float ri = dm / (dp + eps * sgnf(dp + eps));
+                const float pp = fmaxf(0.f, fminf(fminf(b * r, 1.f), fminf(r, b)));
+                const float pm = fmaxf(0.f, fminf(fminf(b * ri, 1.f), fminf(ri, b)));
+                u[i] += -cf * (0.5f * (pp + pm) * fd);
+              }
+            }
+          }
+        }
+      }
+    }
+    if (MODE == COL_RAWNL && live) L.store(d.rout + (long)c * L.nT, u);
+  } else if constexpr (FIN) {
+    if constexpr (MODE == COL_FINAL_SRC) {
+      for (int k = 0; k < d.n_src; ++k) {
+        const float* sp = d.src[k] + (long)c * L.nT;
+#pragma unroll
+        for (int i = 0; i < S; ++i) u[i] += live ? (L.in(i) ? sp[L.t(i)] : 0.f) : 0.f;
+      }
+      if (d.smask && live && d.smask[c]) {
+        const double dl = d.sdelay[c];
+        const double w0 = 2.0 * 3.14159265358979323846 * d.s_f0;
+        const double cexp = 2.0 * d.s_dur;
+#pragma unroll
+        for (int i = 0; i < S; ++i) u[i] += L.in(i) ? tone_burst(d.taxis[L.t(i)] - dl, w0, d.s_ncyc, cexp, d.s_p0) : 0.f;
+      }
+    }
+    // (padding samples of a bounded lane are zero: they add nothing to e2 but are
+    // kept out of the extrema)
+    float mn = BOUNDED ? INFINITY : u[0], mx = BOUNDED ? -INFINITY : u[0];
+    double e2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+      mn = fminf(mn, L.in(i) ? u[i] : INFINITY);
+      mx = fmaxf(mx, L.in(i) ? u[i] : -INFINITY);
+      e2 += (double)u[i] * (double)u[i];
+    }
+    for (int o = LPC / 2; o > 0; o >>= 1) {
+      mn = fminf(mn, __shfl_xor(mn, o, 64));
+      mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+      e2 += __shfl_xor(e2, o, 64);
+    }
+    if (live && row < d.max_rows) {
+      if (L.ln == 0) {
+        const long o = (long)row * ncols + c;
+        const double ib = d.ibefore[c];
+        d.pnp[o] = mn;
+        d.ppp[o] = mx;
+        d.pI[o] = (float)e2;
+        d.pIloss[o] = (float)fmax(0.0, ib - e2);
+        d.ibefore[c] = e2;
+      }
+      if (c == d.cax) L.store(d.pax + (long)row * L.nT, u);
+    }
+    if (live) wmax = fmaxf(wmax, fmaxf(mx, -mn));
+    if (d.rout && live) L.store(d.rout + (long)c * L.nT, u);
+  } else if constexpr (MODE == COL_LOAD) {
+    float mx = 0.f;
+    double e2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+      mx = fmaxf(mx, fabsf(u[i]));
+      e2 += (double)u[i] * (double)u[i];
+    }
+    for (int o = LPC / 2; o > 0; o >>= 1) {
+      mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+      e2 += __shfl_xor(e2, o, 64);
+    }
+    if (live && L.ln == 0 && d.ibefore) d.ibefore[c] = e2;
+    if (live) wmax = fmaxf(wmax, mx);
+  } else if constexpr (MODE == COL_STORE) {
+    if (live) L.store(d.rout + (long)c * L.nT, u);
+  } else if constexpr (MODE == COL_TOREAL) {
+    if (d.taper) {
+#pragma unroll
+      for (int i = 0; i < S; ++i) u[i] *= (L.in(i) ? tt[L.t(i)] : 0.f);
+    }
+    if (live) L.store(d.rout + (long)c * L.nT, u);
+  } else if constexpr (MODE == COL_MAXABS) {
+    // the field the nonlinear pass will see (same transform, same taper), read only
+    if (d.taper) {
+#pragma unroll
+      for (int i = 0; i < S; ++i) u[i] *= (L.in(i) ? tt[L.t(i)] : 0.f);
+    }
+#pragma unroll
+    for (int i = 0; i < S; ++i) wmax = fmaxf(wmax, live ? fabsf(u[i]) : 0.f);
+  }
+}
+
+// True on every lane of the column when any of its rfft bins 0..M is nonzero.
+template <int LPC>
+__device__ __forceinline__ bool spectrum_nonzero(const float2* sc, int M, int ln) {
+  bool nz = false;
+  for (int k = ln; k <= M; k += LPC) {
+    const float2 x = sc[k];
+    nz = nz || x.x != 0.f || x.y != 0.f;
+  }
+  int f = nz ? 1 : 0;
+  for (int o = LPC / 2; o > 0; o >>= 1) f |= __shfl_xor(f, o, 64);
+  return f != 0;
+}
+
+// Time of flight of one column (retarded time; the host adds z / c0), written to
+// d.tof[row][c] by the column's first lane.
+//   TOF_ENVELOPE: u = p, q = its Hilbert transform; the first sample where the
+//     envelope sqrt(u^2 + q^2) reaches tof_ratio of its maximum, interpolated
+//     linearly from the sample before (the first sample itself when it is 0 or
+//     when nothing reaches the threshold).
+//   TOF_MATCHED: u = circular cross-correlation with the reference; its first
+//     maximum k refined by the vertex of the parabola through k-1, k, k+1
+//     (circular), the lag in (-nT/2, nT/2], plus the reference's peak time.
+//   A column without signal (nonzero = false) gets NaN.
+// The values pass through the column's LDS region `lds` (nT floats).
+template <int S, int LPC, bool STRIDED, bool BOUNDED>
+__device__ __forceinline__ void column_tof(const float* u, const float* q, const LaneCol<S, LPC, STRIDED, BOUNDED>& L,
+                                           float* lds, const ColDesc& d, int c, bool live, int ncols, int row,
+                                           bool nonzero) {
+  const bool env = d.tof_mode == TOF_ENVELOPE;
+  const int nT = L.nT;
+  float val[S];
+  float best = -INFINITY;
+  int bidx = 0x7fffffff;
+#pragma unroll
+  for (int i = 0; i < S; ++i) {
+    const float v = env ? sqrtf(u[i] * u[i] + q[i] * q[i]) : u[i];
+    val[i] = L.in(i) ? v : -INFINITY;
+    const int t = L.t(i);
+    if (val[i] > best || (val[i] == best && t < bidx)) { best = val[i]; bidx = t; }
+  }
+  for (int o = LPC / 2; o > 0; o >>= 1) {
+    const float ob = __shfl_xor(best, o, 64);
+    const int oi = __shfl_xor(bidx, o, 64);
+    if (ob > best || (ob == best && oi < bidx)) { best = ob; bidx = oi; }
+  }
+#pragma unroll
+  for (int i = 0; i < S; ++i)
+    if (L.in(i)) lds[L.t(i)] = val[i];
+  int k = bidx;
+  float thr = 0.f;
+  if (env) {
+    thr = d.tof_ratio * best;
+    k = 0x7fffffff;
+#pragma unroll
+    for (int i = 0; i < S; ++i)
+      if (L.in(i) && val[i] >= thr) k = min(k, L.t(i));
+    for (int o = LPC / 2; o > 0; o >>= 1) k = min(k, __shfl_xor(k, o, 64));
+    if (k == 0x7fffffff) k = 0;
+  }
+  wave_sync();
+  if (live && L.ln == 0 && row < d.max_rows) {
+    double res;
+    if (env) {
+      if (k == 0) {
+        res = d.tof_t0;
+      } else {
+        const float e0 = lds[k - 1], e1 = lds[k];
+        const float frac = e1 > e0 ? (thr - e0) / (e1 - e0) : 1.f;
+        res = d.tof_t0 + ((double)(k - 1) + (double)frac) * d.tof_dt;
+      }
+      if (!(best > 0.f)) res = NAN;
+    } else {
+      const float cm = lds[k == 0 ? nT - 1 : k - 1], cp = lds[k == nT - 1 ? 0 : k + 1];
+      const float den = cm - 2.f * best + cp;
+      const float delta = den < 0.f ? 0.5f * (cm - cp) / den : 0.f;
+      double lag = (double)k + (double)delta;
+      if (lag > 0.5 * nT) lag -= nT;
+      res = d.tof_tref + lag * d.tof_dt;
+      if (!nonzero) res = NAN;
+    }
+    d.tof[(long)row * ncols + c] = (float)res;
+  }
+  wave_sync();
+}
+
 template <int M, int MODE, int NLK>
 __global__ __launch_bounds__(256) void k_column(float2* __restrict__ spec, Tables T, Geom g, ColDesc d) {
   using C = ColCfg<M>;
@@ -134,8 +448,13 @@ __global__ __launch_bounds__(256) void k_column(float2* __restrict__ spec, Table
   const int tid = threadIdx.x;
   const int wave = tid / 64, lane = tid % 64;
   const int cw = lane / LPC, ln = lane % LPC;
-  for (int i = tid; i < TWT; i += 256) s_tw[i] = T.twc[i];
-  for (int i = tid; i <= M; i += 256) s_twn[i] = T.twn[i];
+  constexpr bool spectral_in = !(MODE == COL_LOAD || MODE == COL_RAWNL);
+  constexpr bool rewrite = (MODE == COL_NL || MODE == COL_TAPER || MODE == COL_LOAD || MODE == COL_FINAL_SRC);
+  if constexpr (spectral_in || rewrite) {
+    // (COL_RAWNL runs no transform and may be launched without tables)
+    for (int i = tid; i < TWT; i += 256) s_tw[i] = T.twc[i];
+    for (int i = tid; i <= M; i += 256) s_twn[i] = T.twgt[i];  // exp(-2 pi i k / nT), k <= M
+  }
   __syncthreads();
   const float2* twf = s_tw;
   const float2* twi = s_tw + TWF;
@@ -144,41 +463,84 @@ __global__ __launch_bounds__(256) void k_column(float2* __restrict__ spec, Table
   const int per_iter = C::WPB * CPW;
   const int t0 = ln * S;
   const float invN = 1.f / (float)NT_;
-  constexpr bool spectral_in = !(MODE == COL_LOAD || MODE == COL_RAWNL);
   // pointwise modes keep the FFT's strided ownership (lane ln holds samples
   // 2(ln + LPC e) and 2(ln + LPC e) + 1); only the t-stencil modes transpose to
   // contiguous samples
   constexpr bool STRIDED = !(MODE == COL_NL || MODE == COL_RAWNL);
-  auto tsamp = [&](int i) -> int {
-    if constexpr (STRIDED) return 2 * (ln + LPC * (i >> 1)) + (i & 1);
-    else return t0 + i;
-  };
-  // store this lane's samples u[0..S) of a real column starting at dst
-  auto store_real = [&](float* dst, const float* u) {
-    if constexpr (STRIDED) {
-      float2* d2 = reinterpret_cast<float2*>(dst);
-#pragma unroll
-      for (int e = 0; e < E; ++e) d2[ln + LPC * e] = make_float2(u[2 * e], u[2 * e + 1]);
-    } else {
-      float4* d4 = reinterpret_cast<float4*>(dst + t0);
-#pragma unroll
-      for (int q = 0; q < S / 4; ++q) d4[q] = make_float4(u[4 * q], u[4 * q + 1], u[4 * q + 2], u[4 * q + 3]);
-    }
-  };
+  const LaneCol<S, LPC, STRIDED, false> lc{ln, lane, t0, NT_};
   constexpr bool FIN = (MODE == COL_FINAL || MODE == COL_FINAL_SRC);
-  constexpr bool rewrite = (MODE == COL_NL || MODE == COL_TAPER || MODE == COL_LOAD || MODE == COL_FINAL_SRC);
-  const int row = FIN ? *d.row : 0;
-  const float* tt = d.taper == 1 ? T.tapt_h : T.tapt_f;
+  const int row = (FIN || MODE == COL_TOF) ? *d.row : 0;
   // Burgers coefficient, scaled by the beam obliquity when requested (ASS:934)
   const double Nn = d.nscale ? d.N * (d.nscale[0] / (d.nscale[1] + 1e-30)) : d.N;
-  float wmax = 0.f;  // running max|p| of this lane's columns (stats)
+  float wmax = 0.f;  // running max|p| of this lane's columns (stats, COL_MAXABS)
+  // global-max sub-cycling (_kt_flux_adaptive, ASS:842-859): one count for the field
+  int nsub_global = 1;
+  if constexpr (NLK != NL_NONE) {
+    if (d.subcycle == SUBCYCLE_GLOBAL) {
+      const float gm = __uint_as_float(*d.gmax);
+      bool capped;
+      nsub_global = substeps_for(Nn * (double)gm * d.dz / d.dT, (double)d.cfl_target, gm, &capped);
+      if (capped && blockIdx.x == 0 && tid == 0 && d.subcap) atomicAdd(d.subcap, 1u);
+    }
+  }
 
+  const float2* sc_cur = nullptr;  // spectrum row of the current column (null: not live)
+  // irfft of the column's spectrum (bins modified by tof_bin(sel)) into this lane's
+  // samples u[0..S), in the ownership of this mode
+  auto inv_spec = [&](float (&uo)[S], int sel) {
+    float2 v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = ln + LPC * e;
+      float2 x = sc_cur ? sc_cur[k] : make_float2(0.f, 0.f);
+      if (k == 0) x.y = 0.f;
+      x = tof_bin(sel, k, x, M, d.tofref);
+      v[e] = x;
+      cb[pidx(k)] = x;
+    }
+    if (ln == 0) {
+      float2 xn = sc_cur ? sc_cur[M] : make_float2(0.f, 0.f);
+      xn.y = 0.f;
+      xn = tof_bin(sel, M, xn, M, d.tofref);
+      cb[pidx(M)] = xn;
+    }
+    wave_sync();
+    // Z[k] = (X_k + conj X_{M-k}) + i W_N^{-k} (X_k - conj X_{M-k}), scaled by 1/N
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = ln + LPC * e;
+      const float2 a = v[e], b = cconj(cb[pidx(M - k)]);
+      const float2 sm = cadd(a, b), df = csub(a, b);
+      const float2 t = cmul(cconj(s_twn[k]), df);
+      v[e] = make_float2((sm.x - t.y) * invN, (sm.y + t.x) * invN);
+    }
+    wave_sync();
+    cfft<M, true>(v, cb, twi, ln);
+    if constexpr (STRIDED) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) { uo[2 * e] = v[e].x; uo[2 * e + 1] = v[e].y; }
+    } else {
+      // transpose to contiguous ownership: lane ln gets z[E ln .. E ln + E)
+#pragma unroll
+      for (int e = 0; e < E; ++e) cb[pidx(ln + LPC * e)] = v[e];
+      wave_sync();
+#pragma unroll
+      for (int q = 0; q < E; ++q) {
+        const float2 z = cb[pidx(E * ln + q)];
+        uo[2 * q] = z.x;
+        uo[2 * q + 1] = z.y;
+      }
+      wave_sync();
+    }
+  };
   for (int cbase = blockIdx.x * per_iter; cbase < ncols; cbase += gridDim.x * per_iter) {
     const int c = cbase + wave * CPW + cw;
     const bool live = c < ncols && (!d.cflag || d.cflag[c] != 0);
     if (CPW == 1 && !live) continue;  // one column per wave: skip it whole
     float2* sc = spec + (long)c * g.MP;
+    sc_cur = live ? sc : nullptr;
     float u[S];
+    float uq[MODE == COL_TOF ? S : 1];
 
     // ---------------- stage 1: real samples t0 .. t0+S-1 of the column ----------------
     if constexpr (!spectral_in && STRIDED) {
@@ -195,209 +557,24 @@ __global__ __launch_bounds__(256) void k_column(float2* __restrict__ spec, Table
         const float4 x = live ? src[q] : make_float4(0.f, 0.f, 0.f, 0.f);
         u[4 * q] = x.x; u[4 * q + 1] = x.y; u[4 * q + 2] = x.z; u[4 * q + 3] = x.w;
       }
-    } else {
-      float2 v[E];
-#pragma unroll
-      for (int e = 0; e < E; ++e) {
-        const int k = ln + LPC * e;
-        float2 x = live ? sc[k] : make_float2(0.f, 0.f);
-        if (k == 0) x.y = 0.f;
-        v[e] = x;
-        cb[pidx(k)] = x;
-      }
-      if (ln == 0) {
-        float2 xn = live ? sc[M] : make_float2(0.f, 0.f);
-        xn.y = 0.f;
-        cb[pidx(M)] = xn;
-      }
-      wave_sync();
-      // Z[k] = (X_k + conj X_{M-k}) + i W_N^{-k} (X_k - conj X_{M-k}), scaled by 1/N
-#pragma unroll
-      for (int e = 0; e < E; ++e) {
-        const int k = ln + LPC * e;
-        const float2 a = v[e], b = cconj(cb[pidx(M - k)]);
-        const float2 sm = cadd(a, b), df = csub(a, b);
-        const float2 t = cmul(cconj(s_twn[k]), df);
-        v[e] = make_float2((sm.x - t.y) * invN, (sm.y + t.x) * invN);
-      }
-      wave_sync();
-      cfft<M, true>(v, cb, twi, ln);
-      if constexpr (STRIDED) {
-#pragma unroll
-        for (int e = 0; e < E; ++e) { u[2 * e] = v[e].x; u[2 * e + 1] = v[e].y; }
+    } else if constexpr (MODE == COL_TOF) {
+      // the field (envelope: and its Hilbert transform) or its correlation with the reference
+      if (d.tof_mode == TOF_ENVELOPE) {
+        inv_spec(u, 0);
+        inv_spec(uq, 1);
       } else {
-        // transpose to contiguous ownership: lane ln gets z[E ln .. E ln + E)
-#pragma unroll
-        for (int e = 0; e < E; ++e) cb[pidx(ln + LPC * e)] = v[e];
-        wave_sync();
-#pragma unroll
-        for (int q = 0; q < E; ++q) {
-          const float2 z = cb[pidx(E * ln + q)];
-          u[2 * q] = z.x;
-          u[2 * q + 1] = z.y;
-        }
-        wave_sync();
+        inv_spec(u, 2);
       }
+    } else {
+      inv_spec(u, 0);
     }
 
     // ---------------- stage 2: real-space work ----------------
-    if constexpr (MODE == COL_NL || MODE == COL_TAPER || MODE == COL_RAWNL) {
-      if (d.taper) {
-#pragma unroll
-        for (int i = 0; i < S; ++i) u[i] *= tt[tsamp(i)];
-      }
-      if constexpr (NLK != NL_NONE && MODE != COL_TAPER) {
-        int nsub = 1;
-        if (d.subcycle) {
-          float mx = 0.f;
-#pragma unroll
-          for (int i = 0; i < S; ++i) mx = fmaxf(mx, fabsf(u[i]));
-          for (int o = LPC / 2; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
-          const double cfl = Nn * (double)mx * d.dz / d.dT;
-          // ceil(cfl / target) sub-steps above the target, bounded so that a blown-up
-          // column (inf) cannot keep the wave looping; NaN takes one step
-          const double q = cfl / (double)d.cfl_target;
-          if (q > 1.0 && mx >= 1e-30f) nsub = q < (double)MAX_SUBSTEPS ? (int)ceil(q) : MAX_SUBSTEPS;
-        }
-        int nsub_w = nsub;
-        for (int o = 32; o >= LPC; o >>= 1) nsub_w = max(nsub_w, __shfl_xor(nsub_w, o, 64));
-        const float dz = (float)(d.dz / nsub);
-        const float coef = (float)(Nn / d.dT);
-        const int lsrc = lane - 1, rsrc = lane + 1;
-        const bool has_l = ln > 0, has_r = ln < LPC - 1;
-        for (int sub = 0; sub < nsub_w; ++sub) {
-          const bool act = sub < nsub;
-          if constexpr (NLK == NL_KT || NLK == NL_KT_MINMOD) {
-            float h[S + 4], r0[S], s1[S];
-            // halo: 2 samples from each neighbour lane (zero outside the column)
-            {
-              const float a = shf(u[S - 2], lsrc), b = shf(u[S - 1], lsrc);
-              const float cc = shf(u[0], rsrc), dd = shf(u[1], rsrc);
-              h[0] = has_l ? a : 0.f; h[1] = has_l ? b : 0.f;
-              h[S + 2] = has_r ? cc : 0.f; h[S + 3] = has_r ? dd : 0.f;
-#pragma unroll
-              for (int i = 0; i < S; ++i) h[i + 2] = u[i];
-            }
-            kt_rhs_local<S, NLK == NL_KT>(h, t0, NT_, coef, r0);
-#pragma unroll
-            for (int i = 0; i < S; ++i) s1[i] = act ? u[i] + dz * r0[i] : u[i];
-            {
-              const float a = shf(s1[S - 2], lsrc), b = shf(s1[S - 1], lsrc);
-              const float cc = shf(s1[0], rsrc), dd = shf(s1[1], rsrc);
-              h[0] = has_l ? a : 0.f; h[1] = has_l ? b : 0.f;
-              h[S + 2] = has_r ? cc : 0.f; h[S + 3] = has_r ? dd : 0.f;
-#pragma unroll
-              for (int i = 0; i < S; ++i) h[i + 2] = s1[i];
-            }
-            kt_rhs_local<S, NLK == NL_KT>(h, t0, NT_, coef, r0);
-            if (act) {
-#pragma unroll
-              for (int i = 0; i < S; ++i) u[i] = 0.5f * u[i] + 0.5f * (s1[i] + dz * r0[i]);
-            }
-          } else {
-            float h[S + 2];
-            {
-              const float a = shf(u[S - 1], lsrc), b = shf(u[0], rsrc);
-              h[0] = has_l ? a : 0.f;
-              h[S + 1] = has_r ? b : 0.f;
-#pragma unroll
-              for (int i = 0; i < S; ++i) h[i + 1] = u[i];
-            }
-            const float cf = (float)(Nn * (double)dz / d.dT);
-            float fl[S + 1];
-#pragma unroll
-            for (int i = 0; i < S + 1; ++i) {
-              const float uL = h[i], uR = h[i + 1];
-              const float a = fmaxf(fabsf(uL), fabsf(uR));
-              if constexpr (NLK == NL_RUSANOV) fl[i] = -(uL * uL + uR * uR) / 2.f - a * (uR - uL);
-              else fl[i] = 0.5f * (-0.5f * (uL * uL + uR * uR)) - 0.5f * a * (uR - uL);
-            }
-#pragma unroll
-            for (int i = 0; i < S; ++i) {
-              const int t = t0 + i;
-              if (act && t >= 1 && t <= NT_ - 2) {
-                const float fd = fl[i + 1] - fl[i];
-                if constexpr (NLK == NL_RUSANOV) {
-                  u[i] += -cf * fd;
-                } else {
-                  const float dm = h[i + 1] - h[i], dp = h[i + 2] - h[i + 1];
-                  const float eps = 1e-30f, b = d.beta_tvd;
-                  const float r = dp / (dm + eps * sgnf(dm + eps));
-                  const float ri = dm / (dp + eps * sgnf(dp + eps));
-                  const float pp = fmaxf(0.f, fminf(fminf(b * r, 1.f), fminf(r, b)));
-                  const float pm = fmaxf(0.f, fminf(fminf(b * ri, 1.f), fminf(ri, b)));
-                  u[i] += -cf * (0.5f * (pp + pm) * fd);
-                }
-              }
-            }
-          }
-        }
-      }
-      if (MODE == COL_RAWNL && live) store_real(d.rout + (long)c * NT_, u);
-    } else if constexpr (FIN) {
-      if constexpr (MODE == COL_FINAL_SRC) {
-        for (int k = 0; k < d.n_src; ++k) {
-          const float* sp = d.src[k] + (long)c * NT_;
-#pragma unroll
-          for (int i = 0; i < S; ++i) u[i] += live ? sp[tsamp(i)] : 0.f;
-        }
-        if (d.smask && live && d.smask[c]) {
-          const double dl = d.sdelay[c];
-          const double w0 = 2.0 * 3.14159265358979323846 * d.s_f0;
-          const double cexp = 2.0 * d.s_dur;
-#pragma unroll
-          for (int i = 0; i < S; ++i) u[i] += tone_burst(d.taxis[tsamp(i)] - dl, w0, d.s_ncyc, cexp, d.s_p0);
-        }
-      }
-      float mn = u[0], mx = u[0];
-      double e2 = 0.0;
-#pragma unroll
-      for (int i = 0; i < S; ++i) {
-        mn = fminf(mn, u[i]);
-        mx = fmaxf(mx, u[i]);
-        e2 += (double)u[i] * (double)u[i];
-      }
-      for (int o = LPC / 2; o > 0; o >>= 1) {
-        mn = fminf(mn, __shfl_xor(mn, o, 64));
-        mx = fmaxf(mx, __shfl_xor(mx, o, 64));
-        e2 += __shfl_xor(e2, o, 64);
-      }
-      if (live && row < d.max_rows) {
-        if (ln == 0) {
-          const long o = (long)row * ncols + c;
-          const double ib = d.ibefore[c];
-          d.pnp[o] = mn;
-          d.ppp[o] = mx;
-          d.pI[o] = (float)e2;
-          d.pIloss[o] = (float)fmax(0.0, ib - e2);
-          d.ibefore[c] = e2;
-        }
-        if (c == d.cax) store_real(d.pax + (long)row * NT_, u);
-      }
-      if (live) wmax = fmaxf(wmax, fmaxf(mx, -mn));
-      if (d.rout && live) store_real(d.rout + (long)c * NT_, u);
-    } else if constexpr (MODE == COL_LOAD) {
-      float mx = 0.f;
-      double e2 = 0.0;
-#pragma unroll
-      for (int i = 0; i < S; ++i) {
-        mx = fmaxf(mx, fabsf(u[i]));
-        e2 += (double)u[i] * (double)u[i];
-      }
-      for (int o = LPC / 2; o > 0; o >>= 1) {
-        mx = fmaxf(mx, __shfl_xor(mx, o, 64));
-        e2 += __shfl_xor(e2, o, 64);
-      }
-      if (live && ln == 0 && d.ibefore) d.ibefore[c] = e2;
-      if (live) wmax = fmaxf(wmax, mx);
-    } else if constexpr (MODE == COL_STORE) {
-      if (live) store_real(d.rout + (long)c * NT_, u);
-    } else if constexpr (MODE == COL_TOREAL) {
-      if (d.taper) {
-#pragma unroll
-        for (int i = 0; i < S; ++i) u[i] *= tt[tsamp(i)];
-      }
-      if (live) store_real(d.rout + (long)c * NT_, u);
+    if constexpr (MODE == COL_TOF) {
+      const bool nz = spectrum_nonzero<LPC>(sc, live ? M : -1, ln);
+      column_tof<S, LPC, STRIDED, false>(u, uq, lc, reinterpret_cast<float*>(cb), d, c, live, ncols, row, nz);
+    } else {
+      column_realspace<S, LPC, STRIDED, false, MODE, NLK>(u, lc, d, T, c, live, ncols, row, Nn, nsub_global, wmax);
     }
 
     // ---------------- stage 3: rfft back into the spectrum ----------------
@@ -433,17 +610,22 @@ __global__ __launch_bounds__(256) void k_column(float2* __restrict__ spec, Table
     }
   }
 
-  // max|p| of the block -> stats slot (one atomic per block)
-  if constexpr (FIN || MODE == COL_LOAD) {
-    if (!d.stats) return;
+  // max|p| of the block -> stats slot (one atomic per block), or for COL_MAXABS
+  // -> this block's slot of the partial maxima (no atomics: a reduction kernel follows)
+  if constexpr (FIN || MODE == COL_LOAD || MODE == COL_MAXABS) {
+    if (MODE != COL_MAXABS && !d.stats) return;
     for (int o = 32; o > 0; o >>= 1) wmax = fmaxf(wmax, __shfl_xor(wmax, o, 64));
     if (lane == 0) s_red[wave] = wmax;
     __syncthreads();
     if (tid == 0) {
       float m = s_red[0];
       for (int k = 1; k < C::WPB; ++k) m = fmaxf(m, s_red[k]);
-      unsigned* slot = FIN ? d.stats + row + 1 : d.stats;
-      if (!FIN || row < d.max_rows) atomicMax(slot, __float_as_uint(m));
+      if constexpr (MODE == COL_MAXABS) {
+        d.partial[blockIdx.x] = __float_as_uint(m);
+      } else {
+        unsigned* slot = FIN ? d.stats + row + 1 : d.stats;
+        if (!FIN || row < d.max_rows) atomicMax(slot, __float_as_uint(m));
+      }
     }
   }
 }

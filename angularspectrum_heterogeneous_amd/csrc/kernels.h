@@ -130,13 +130,18 @@ struct Tables {
   const float* sa_cw;    // [MP] c0/omega_m (0 where the coefficient is zeroed)
   const float2* twx;     // [nX] exp(-2 pi i k / nX)
   const float2* twy;     // [nY]
-  const float2* twm;     // [M]  exp(-2 pi i k / M)
-  const float2* twn;     // [M+1] exp(-2 pi i k / nT)
   const float2* twc;     // column-transform pass twiddles: forward passes, then inverse passes
+  const float2* twgx;    // [nX] exp(-2 pi i k / nX), general-size x kernels
+  const float2* twgy;    // [nY]
+  const float2* twgt;    // [nT] exp(-2 pi i k / nT): column packing, general-length column kernel
 };
 
 struct Geom {
-  int nX, nY, nT, M, MP, nfb;
+  int nX, nY, nT;
+  int M;    // last rfft bin, nT / 2
+  int MP;   // spectrum row length (>= M + 1, multiple of LF)
+  int nfb;  // LF-bin tiles per row
+  int nyq;  // the Nyquist bin (M) for even nT, -1 for odd nT
 };
 
 // Operator descriptor for one k-space visit.
@@ -174,6 +179,20 @@ __device__ __forceinline__ void bounded_sincos(float x, float* sn, float* cs) {
   const float c1 = (q & 1) ? s : c;
   *sn = (q & 2) ? -s1 : s1;
   *cs = ((q + 1) & 2) ? -c1 : c1;
+}
+
+// kx^2 + ky^2 and k^2 - kk with every product and sum rounded on its own, as
+// NumPy evaluates them (ASS:492, ASS:506, ASS:608, ASS:620).  A fused
+// multiply-add rounds once and can move the result by an ulp; on grids where a
+// mode sits exactly at the evanescent cutoff k_z = 0 that ulp flips the mode
+// between the attenuation's obliquity factor exp(-alpha dz 1e6) and 1.
+__device__ __forceinline__ double kperp2(double kx, double ky) {
+#pragma clang fp contract(off)
+  return kx * kx + ky * ky;
+}
+__device__ __forceinline__ double kz2_of(double k, double kk) {
+#pragma clang fp contract(off)
+  return k * k - kk;
 }
 
 // Per-f constants of one visit, staged in LDS by the y-pass.
@@ -353,12 +372,11 @@ __global__ __launch_bounds__(LNT) void k_ypass(float2* __restrict__ spec, Tables
   using LY = Lay<L, LFY, 0, true>;
   {
     const double kx = T.kx[ip];
-    const double kx2 = kx * kx;
     // entry e = tid + LNT k is on line e % LF = myline
 #pragma unroll 2
     for (int e = tid; e < H * LFY; e += LNT) {
       const double ky = T.ky[e / LFY];
-      const double kk = kx2 + ky * ky;
+      const double kk = kperp2(kx, ky);
       if constexpr (PRECISE) opt[e] = visit_operator_fp64(op, kk, lc, f0 + myline);
       else opt[e] = visit_operator(op, kk, lc, f0 + myline);
     }
@@ -458,7 +476,7 @@ __global__ __launch_bounds__(LNT) void k_xfft_inv(float2* __restrict__ spec, Tab
     auto st_lds = [&](int line, int pos, float2 v) {
       const int m = f0 + line;
       v = cscale(v, e.scale);
-      if (m == 0 || m == g.M) v.y = 0.f;  // irfft/rfft round trip of the reference
+      if (m == 0 || m == g.nyq) v.y = 0.f;  // irfft/rfft round trip of the reference
       if (e.fw) v = cscale(v, exp2f(sfwe[line] * (lgx[pos] + lgy)));
       buf[LY::idx(line, pos)] = v;
     };
@@ -562,7 +580,7 @@ __global__ __launch_bounds__(LNT) void k_ysa(float2* __restrict__ spec, Tables T
         u = csub(u, cscale(pin, s.sa_strength * w));
       }
     }
-    if (m == 0 || m == g.M) u.y = 0.f;
+    if (m == 0 || m == g.nyq) u.y = 0.f;
     if (s.phase) {
       const long idx = (long)x * g.nY + pos;
       const float fs = (float)m / s.f0b;
@@ -574,7 +592,7 @@ __global__ __launch_bounds__(LNT) void k_ysa(float2* __restrict__ spec, Tables T
         const float a = fminf(fmaxf(s.amp[idx], 1e-6f), 1.f);
         u = cscale(u, powf(a, fs));
       }
-      if (m == 0 || m == g.M) u.y = 0.f;
+      if (m == 0 || m == g.nyq) u.y = 0.f;
     }
     res[q] = u;
   }
@@ -599,7 +617,15 @@ enum ColMode : int {
   COL_RAWNL = 5,   // real field in -> nonlinear step -> real field out (no FFT)
   COL_FINAL_SRC = 6,  // COL_FINAL on a step that injects source slices
   COL_TOREAL = 7,  // spectrum -> taper -> real field (the learned step's input)
+  COL_MAXABS = 8,  // spectrum -> taper -> per-block max|p| into ColDesc::partial (read only)
+  COL_TOF = 9,     // spectrum -> time of flight of the column into ColDesc::tof (read only)
 };
+
+// Time-of-flight estimators (ColDesc::tof_mode)
+enum TofMode : int { TOF_OFF = 0, TOF_ENVELOPE = 1, TOF_MATCHED = 2 };
+
+// CFL sub-cycling of the KT update (ColDesc::subcycle)
+enum Subcycle : int { SUBCYCLE_OFF = 0, SUBCYCLE_COLUMN = 1, SUBCYCLE_GLOBAL = 2 };
 
 enum NLKind : int { NL_NONE = 0, NL_KT = 1, NL_RUSANOV = 2, NL_TVD = 3, NL_KT_MINMOD = 4 };
 
@@ -607,8 +633,18 @@ struct ColDesc {
   int mode;
   int nl;            // NLKind
   int taper;         // 0 none, 1 sqrt taper (split), 2 full taper (sequential)
-  int subcycle;      // 0 off, 1 per column
+  int subcycle;      // Subcycle
   float cfl_target;
+  const unsigned* gmax;  // SUBCYCLE_GLOBAL: float bits of the field's max|p|
+  unsigned* subcap;      // when set, += 1 per column (or field) whose sub-steps hit MAX_SUBSTEPS
+  unsigned* partial;     // COL_MAXABS: per-block max|p| bits, [gridDim.x]
+  // time of flight (COL_TOF): retarded arrival time per column into tof[row][c]
+  float* tof;
+  int tof_mode;          // TofMode
+  float tof_ratio;       // envelope threshold fraction
+  double tof_t0, tof_dt; // time of sample 0 and the sample spacing of the time axis
+  double tof_tref;       // matched filter: time at which the reference trace peaks
+  const float2* tofref;  // matched filter: rfft of the reference trace [MP]
   double N, dT, dz;  // Burgers coefficient, time step, z step of the nonlinear update
   float beta_tvd;
   // real-field io
@@ -652,6 +688,16 @@ __global__ __launch_bounds__(256) void k_source_field(const unsigned char* __res
     const int t = (int)(e - c * nT);
     out[e] = mask[c] ? tone_burst(taxis[t] - delay[c], w0, ncyc, cexp, p0) : 0.f;
   }
+}
+
+// Spectrum bin fed to the inverse transform of a column pass: the field itself
+// (sel 0), its Hilbert transform (sel 1: -i X on bins 1..ceil(nT/2)-1, the DC and
+// Nyquist bins dropped) or its circular cross-correlation with the reference
+// trace (sel 2: X conj(R)).
+__device__ __forceinline__ float2 tof_bin(int sel, int k, float2 x, int nyq, const float2* ref) {
+  if (sel == 1) return (k == 0 || k == nyq) ? make_float2(0.f, 0.f) : make_float2(x.y, -x.x);
+  if (sel == 2) return cmul(x, cconj(ref[k]));
+  return x;
 }
 
 // MC / minmod slope of cell i from its neighbours (ASS:727-755); float32 literal ops
@@ -721,6 +767,44 @@ __device__ __forceinline__ void kt_rhs_local(const float* u, int t0, int nT, flo
 namespace ash {
 
 __global__ void k_set_row(int* row, int v) { *row = v; }
+__global__ void k_set_u32(unsigned* a, unsigned v) { *a = v; }
+__global__ void k_copy_u32x2(const unsigned* a, const unsigned* b, unsigned* out) { out[0] = *a; out[1] = *b; }
+
+// Partial maxima of |u| over a real field of ncols columns of nT samples: block b
+// covers columns [64 b, 64 b + 64) and writes partial[b] (float bits).  The
+// maximum is taken on the bit patterns of |u|, which order like the values and
+// put a NaN above +inf, so a NaN sample shows in the result.
+constexpr int ABSMAX_COLS = 64;
+__global__ __launch_bounds__(256) void k_absmax_cols(const float* __restrict__ u, long ncols, int nT,
+                                                     unsigned* __restrict__ partial) {
+  __shared__ unsigned s_red[4];
+  const long c0 = (long)blockIdx.x * ABSMAX_COLS;
+  const long c1 = c0 + ABSMAX_COLS < ncols ? c0 + ABSMAX_COLS : ncols;
+  const long e0 = c0 * nT, e1 = c1 * nT;
+  unsigned m = 0u;
+  for (long e = e0 + threadIdx.x; e < e1; e += 256) m = max(m, __float_as_uint(fabsf(u[e])));
+  for (int o = 32; o > 0; o >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, o, 64));
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = max(max(s_red[0], s_red[1]), max(s_red[2], s_red[3]));
+}
+
+// out[0] = max of n partial maxima (float bits; all finite or +inf values are
+// non-negative floats, whose bit patterns order like the values).  One block.
+__global__ __launch_bounds__(1024) void k_reduce_max(const unsigned* __restrict__ partial, int n,
+                                                     unsigned* __restrict__ out) {
+  __shared__ unsigned s_red[16];
+  unsigned m = 0u;
+  for (int i = threadIdx.x; i < n; i += 1024) m = max(m, partial[i]);
+  for (int o = 32; o > 0; o >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, o, 64));
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned r = s_red[0];
+    for (int w = 1; w < 16; ++w) r = max(r, s_red[w]);
+    out[0] = r;
+  }
+}
 __global__ void k_zero2(double* a) { a[0] = 0.0; a[1] = 0.0; }
 
 // y transform of one (kx row, f block) tile and the power-weighted obliquity
@@ -756,7 +840,7 @@ __global__ __launch_bounds__(LNT) void k_ypower(const float2* __restrict__ spec,
   if (km > 0.0) {
     for (int pos = tid / LF; pos < L; pos += LNT / LF) {
       const double ky = T.ky[pos];
-      const double kz2 = km * km - (kx * kx + ky * ky);
+      const double kz2 = kz2_of(km, kperp2(kx, ky));
       if (kz2 > 0.0) {
         const float2 v = buf[LY::idx(line, pos)];
         const double P = (double)v.x * v.x + (double)v.y * v.y;

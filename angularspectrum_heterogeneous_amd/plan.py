@@ -38,7 +38,12 @@ def desc_from_params(shape, params, subcycle: str = "off", cfl_target: float = 0
     d.superAbsorbingStrength = float(params.superAbsorbingStrength)
     d.fluxScheme = 1 if params.fluxScheme == "kt" else 0
     d.useObliquityCorrection = int(bool(params.useObliquityCorrection))
-    d.subcycle = {"off": _lib.SUBCYCLE_OFF, "column": _lib.SUBCYCLE_COLUMN}[subcycle]
+    modes = {"off": _lib.SUBCYCLE_OFF, "column": _lib.SUBCYCLE_COLUMN, "global": _lib.SUBCYCLE_GLOBAL}
+    if subcycle not in modes:
+        raise ValueError(f"ash_subcycle must be one of {sorted(modes)}, got {subcycle!r}")
+    if subcycle != "off" and not (params.useSplitStep and params.fluxScheme == "kt"):
+        raise ValueError("CFL sub-cycling applies to the split-step KT update only (fluxScheme='kt')")
+    d.subcycle = modes[subcycle]
     d.cflTarget = float(cfl_target)
     d.useNonlinearityObliquity = int(bool(params.useNonlinearityObliquity))
     return d
@@ -73,6 +78,12 @@ class MarchPlan:
         self.net = net
         check(self.lib.ash_plan_set_learned(self.h, net.h if net is not None else None, float(dpdt_max)),
               "ash_plan_set_learned")
+
+    def set_tof(self, mode: int, env_ratio: float, t0: float, dt: float, ref=None, t_ref_peak: float = 0.0):
+        """Time of flight after every step (ash_plan_set_tof); ref: (nT,) float32 CUDA tensor."""
+        self._tof_ref = ref
+        check(self.lib.ash_plan_set_tof(self.h, int(mode), float(env_ratio), float(t0), float(dt), ptr(ref),
+                                        float(t_ref_peak), _lib.current_stream()), "ash_plan_set_tof")
 
     def autotune(self) -> dict:
         """Times the radix orders and the column grid on this plan and keeps the fastest."""
@@ -134,10 +145,19 @@ class MarchPlan:
         return out
 
     def nonlinear_step(self, x, scheme: int, dz: float, beta_tvd: float = 1.0, subcycle: int = 0, out=None):
+        if subcycle != _lib.SUBCYCLE_OFF and scheme not in (_lib.NL_KT, _lib.NL_KT_MINMOD):
+            raise ValueError("CFL sub-cycling applies to the KT schemes only")
         out = self._io(x, out)
         check(self.lib.ash_nonlinear_step(self.h, ptr(x), ptr(out), int(scheme), float(dz), float(beta_tvd),
                                           int(subcycle), _lib.current_stream()), "ash_nonlinear_step")
         return out
+
+    def subcycle_capped(self) -> int:
+        """Columns (or fields) whose sub-cycling hit the kernels' bound of
+        MAX_SUBSTEPS sub-steps since the last load (synchronises)."""
+        n = ctypes.c_int64()
+        check(self.lib.ash_plan_subcycle_capped(self.h, ctypes.byref(n)), "ash_plan_subcycle_capped")
+        return int(n.value)
 
     def max_abs(self, x) -> float:
         _check_field(x, self.shape)

@@ -12,8 +12,10 @@ reference, all deliberate:
   (restarts are replayed exactly as the reference does when dZ does change);
 * ``per_step_callback`` receives a torch tensor on the GPU instead of a JAX
   array;
-* TOF extraction raises ``NotImplementedError``: the reference imports it from
-  a module (``tof_extraction``) that it does not ship (ASS:26-29);
+* TOF extraction (``tof_ref_trace`` / ``tof_env_ratio``) runs on the GPU with
+  this library's estimators: the reference imports them from a module
+  (``tof_extraction``, ASS:26-29) that it does not ship, so they follow its
+  docstring (ASS:1244-1264; see ``ash_plan_set_tof``); ``taxis`` must be uniform;
 * ``useNonlinearityObliquity=True`` together with the learned nonlinear step
   (``ash_learned``) raises ``ValueError``;
 * an initial field or source plane given as a ``SourceSlice`` (the sparse
@@ -72,7 +74,7 @@ class SolverParams:
     phaseScreens: object = None
     sourcePlanes: object = None
     # --- extensions ---
-    ash_subcycle: str = 'off'       # 'off' | 'column': per-column CFL sub-cycling of KT
+    ash_subcycle: str = 'off'       # 'off' | 'global' (_kt_flux_adaptive) | 'column': CFL sub-cycling of KT
     ash_cfl_target: float = 0.5
     ash_learned: object = None      # a learned.LearnedStep: network update on smooth columns
     ash_dpdt_max: float = 6e12      # KT on columns whose max|dp/dt| exceeds this (Pa/s)
@@ -80,6 +82,11 @@ class SolverParams:
 
 class StabilityError(RuntimeError):
     """The stability criterion failed and dZ cannot shrink (the reference loops forever)."""
+
+
+class SubcycleLimitError(RuntimeError):
+    """CFL sub-cycling needed more sub-steps per dZ than the kernels take (1024);
+    the reference's host loop has no bound."""
 
 
 def _as_device_field(x) -> torch.Tensor:
@@ -121,15 +128,19 @@ def angular_spectrum_solve(
     Same contract as ASS:1217-1567; returns
     ``(field, pnp, ppp, pI, pIloss, zaxis, pax)`` as NumPy arrays:
     field (nX,nY,nT) float32; pnp/ppp/pI/pIloss (nX,nY,nZ) float32;
-    zaxis (nZ,) float64; pax (nT,nZ) float32.
+    zaxis (nZ,) float64; pax (nT,nZ) float32; with TOF enabled (``tof_ref_trace``,
+    which takes precedence, or ``tof_env_ratio``) an eighth output, tof
+    (nX,nY,nZ) float32, the arrival time (retarded time + z / c0) per column.
     """
+    tof_cfg = None
     if tof_ref_trace is not None or tof_env_ratio is not None:
         if tof_ref_trace is not None and tof_t_ref_peak_s is None:
             raise ValueError('tof_t_ref_peak_s is required when tof_ref_trace is provided')
-        raise NotImplementedError(
-            'TOF extraction needs the tof_extraction module, which the reference does not ship')
+        tof_cfg = (tof_ref_trace, tof_t_ref_peak_s, tof_env_ratio, taxis)
     if params.useNonlinearityObliquity and params.ash_learned is not None:
         raise ValueError('the learned nonlinear step does not take the beam obliquity scale')
+    if params.ash_subcycle != 'off' and params.ash_learned is not None:
+        raise ValueError('the learned nonlinear step replaces the un-sub-cycled KT update only')
 
     field0 = _as_device_field(initial_field)
     nX, nY, nT = field0.shape
@@ -141,6 +152,8 @@ def angular_spectrum_solve(
     dZ = max(dZ, params.dZmin)
 
     plan = MarchPlan((nX, nY, nT), params, params.ash_subcycle, params.ash_cfl_target)
+    if tof_cfg is not None:
+        _configure_tof(plan, tof_cfg, nT, params.dT)
     if params.ash_learned is not None:
         plan.set_learned(params.ash_learned, params.ash_dpdt_max)
     if _plan_out is not None:
@@ -148,11 +161,32 @@ def angular_spectrum_solve(
     screens = _upload_screens(params.phaseScreens)
     while True:
         try:
-            return _march(plan, field0, params, N, dZ, verbose, per_step_callback, screens, check_every)
+            return _march(plan, field0, params, N, dZ, verbose, per_step_callback, screens, check_every,
+                          tof_cfg is not None)
         except _Restart as r:
             if verbose:
                 print('Stability criterion violated — reducing step size')
             dZ = r.dz
+
+
+def _configure_tof(plan: MarchPlan, cfg, nT: int, dT: float) -> None:
+    """TOF on the plan (ASS:1387-1397): matched filter when a reference trace is
+    given, else the envelope threshold; the time axis must be uniform."""
+    ref, t_ref_peak, env_ratio, taxis = cfg
+    tax = np.asarray(taxis, dtype=np.float64) if taxis is not None else np.arange(nT, dtype=np.float64) * dT
+    if tax.shape != (nT,):
+        raise ValueError(f'taxis must have nT = {nT} samples')
+    dt = float(tax[1] - tax[0]) if nT > 1 else float(dT)
+    if nT > 2 and np.max(np.abs(np.diff(tax) - dt)) > 1e-9 * abs(dt):
+        raise ValueError('TOF extraction needs a uniform time axis')
+    if ref is not None:
+        r = np.asarray(ref, dtype=np.float32)
+        if r.shape != (nT,):
+            raise ValueError(f'tof_ref_trace must have nT = {nT} samples')
+        plan.set_tof(_lib.TOF_MATCHED, 1.0, float(tax[0]), dt, torch.from_numpy(r).to('cuda'),
+                     float(t_ref_peak))
+    else:
+        plan.set_tof(_lib.TOF_ENVELOPE, float(env_ratio), float(tax[0]), dt)
 
 
 def _upload_screens(screens):
@@ -175,7 +209,8 @@ class _Marcher:
     steps, callback) and ``result`` returns the outputs.  Used by the single
     solve and by the batched solve."""
 
-    def __init__(self, plan: MarchPlan, field0, p: SolverParams, N, dZ, verbose, cb, screens, check_every):
+    def __init__(self, plan: MarchPlan, field0, p: SolverParams, N, dZ, verbose, cb, screens, check_every,
+                 tof: bool = False):
         nX, nY, nT = field0.shape
         self.plan, self.p, self.N, self.dZ = plan, p, N, dZ
         self.verbose, self.cb, self.screens, self.check_every = verbose, cb, screens, check_every
@@ -186,6 +221,7 @@ class _Marcher:
         self.pax = torch.zeros((max_steps, nT), dtype=torch.float32, device=dev)
         self.stats = torch.zeros(max_steps + 1, dtype=torch.int32, device=dev)
         self.field_cb = torch.empty((nX, nY, nT), dtype=torch.float32, device=dev) if cb is not None else None
+        self.tof = torch.zeros((max_steps, nX, nY), dtype=torch.float32, device=dev) if tof else None
         plan.load(field0, self.stats)
         io = self.io = AshStepIO()
         io.max_rows = max_steps
@@ -193,6 +229,7 @@ class _Marcher:
         io.pax_dev = self.pax.data_ptr()
         io.stats_dev = self.stats.data_ptr()
         io.field_out_dev = self.field_cb.data_ptr() if self.field_cb is not None else None
+        io.tof_dev = self.tof.data_ptr() if self.tof is not None else None
         self.keep = []      # device copies of sources crossed in the current step
         self.zvec = []
         self.z = 0.0        # running sum(zvec) (ASS:1413-1415 recomputes it every step)
@@ -207,6 +244,12 @@ class _Marcher:
     def _check_stability(self, upto):
         if upto <= self.checked:
             return
+        if self.plan.desc.subcycle != _lib.SUBCYCLE_OFF:
+            n_capped = self.plan.subcycle_capped()
+            if n_capped:
+                raise SubcycleLimitError(
+                    f'CFL sub-cycling needed more than {_lib.MAX_SUBSTEPS} sub-steps per dZ in '
+                    f'{n_capped} column(s) or field(s) by step {upto - 1}')
         p, N = self.p, self.N
         mx = self.stats[self.checked:upto].cpu().view(torch.float32).numpy().astype(np.float64)
         for k in range(upto - self.checked):
@@ -295,11 +338,16 @@ class _Marcher:
         res = _to_host(self.outs[:, :cc].permute(0, 2, 3, 1).contiguous())
         pax_np = _to_host(self.pax[:cc].t().contiguous())
         zaxis = np.cumsum(self.zvec[:cc])
-        return _to_host(field), res[0], res[1], res[2], res[3], zaxis, pax_np
+        out = (_to_host(field), res[0], res[1], res[2], res[3], zaxis, pax_np)
+        if self.tof is not None:
+            # retarded arrival time + z / c0 (the march's time axis travels with the pulse)
+            tof = _to_host(self.tof[:cc].permute(1, 2, 0).contiguous()).astype(np.float64)
+            out += ((tof + zaxis[None, None, :] / self.p.c0).astype(np.float32),)
+        return out
 
 
-def _march(plan: MarchPlan, field0, p: SolverParams, N, dZ, verbose, cb, screens, check_every):
-    m = _Marcher(plan, field0, p, N, dZ, verbose, cb, screens, check_every)
+def _march(plan: MarchPlan, field0, p: SolverParams, N, dZ, verbose, cb, screens, check_every, tof=False):
+    m = _Marcher(plan, field0, p, N, dZ, verbose, cb, screens, check_every, tof)
     while not m.done:
         io = m.prepare()
         plan.step(io)

@@ -10,10 +10,13 @@ stays resident in HBM.  value = nX*nY*nT*steps / wall time of the timed steps.
 Also reported:
   roofline      - the dominant kernel's algorithmic HBM bytes per launch divided
                   by its average launch time (HIP events on the launch stream);
-  step_roofline - the step's algorithmic bytes (104 B per voxel-z-step) at the
-                  measured step rate, as a fraction of 8 TB/s;
+  step_roofline - the step's algorithmic bytes (the sum over the kernels launched
+                  per step) at the measured step rate, as a fraction of 8 TB/s,
+                  and the same with the measured HBM bytes per step (rocprofv3
+                  FETCH_SIZE + WRITE_SIZE, calibrated) when the profile is present;
   cpu_baseline  - the FP32 NumPy/SciPy restatement of the reference march timed
-                  on this host's cores over a bounded sample of the same workload;
+                  on this host's cores: the 64x64x256 march in full, and 5 steps of
+                  this workload after one warm-up step;
   batch         - 32 seeded transmit configurations of uneven size (nX=nY in
                   {128,192,256,384}, nT in {512,1024,2048}) solved as one batch
                   with autotuned launches, against the same solves one by one;
@@ -38,36 +41,60 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "voxel-z-steps/s at 256x256x1024 FP32; learned vs plain KT at rel-L2<=1e-3"
 
 
-def algorithmic_bytes_per_launch(kind: str, vox: int, ncols: int, nT: int) -> float:
+def algorithmic_bytes_per_launch(kind: str, vox: int, ncols: int, nT: int, nX: int = 0, band_rows: int = 0) -> float:
     """HBM bytes a kernel must move: one read + one write of the field state
     (4 B + 4 B per voxel: the complex64 half spectrum has the real field's size)
     for every sweep; the final column kernel only reads it and writes the
-    per-column reductions (4 x 4 B) and the FP64 intensity carry (8 B r+w)."""
+    per-column reductions (4 x 4 B) and the FP64 intensity carry (8 B r+w); the
+    super-absorbing y pass reads and writes only its two boundary bands
+    (band_rows of the nY rows of every x row)."""
     if kind == "column_final":
         return 4.0 * vox + ncols * (16 + 16) + 4 * nT
+    if kind == "ysa" and band_rows:
+        return 8.0 * vox * band_rows / (ncols // nX)
     return 8.0 * vox
 
 
 def cpu_baseline(steps: int):
+    """The FP32 NumPy/SciPy restatement of the reference march on this host: the
+    64x64x256 march in full, then `steps` steady-state steps of the 256x256x1024
+    march after one warm-up step (operator setup timed separately)."""
+    import platform
     import numpy as np
     from oracle import asm_oracle as O
     from angularspectrum_heterogeneous_amd import workloads as W
     cores = min(16, len(os.sched_getaffinity(0)))
     O.set_workers(cores)
+    f1, kw1 = W.config1()
+    tim1 = {}
+    t0 = time.perf_counter()
+    res1 = O.angular_spectrum_solve(f1, O.OracleParams(**kw1), dtype=np.float32, timings=tim1)
+    wall1 = time.perf_counter() - t0
+    n1 = len(res1[5])
     field, kw = W.config2()
-    p = O.OracleParams(**kw)
     tim = {}
-    O.angular_spectrum_solve(field, p, dtype=np.float32, max_steps_run=steps, timings=tim)
-    t = sum(tim["step_s"])
+    O.angular_spectrum_solve(field, O.OracleParams(**kw), dtype=np.float32, max_steps_run=steps + 1, timings=tim)
+    t = sum(tim["step_s"][1:])
     vox = field.size
+    cpu = platform.processor() or platform.machine()
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {"value": vox * steps / t, "unit": "voxel-z-steps/s", "cores": cores, "kind": "port",
-            "sample": f"{steps} z-steps of the same 256x256x1024 march by the FP32 NumPy/SciPy "
-                      f"restatement of the reference (scipy.fft, {cores} workers); operator "
+            "sample": f"{steps} z-steps of the same 256x256x1024 march after one warm-up step, by the FP32 "
+                      f"NumPy/SciPy restatement of the reference (scipy.fft, {cores} workers); operator "
                       f"setup ({tim['setup_s']:.1f} s) excluded",
-            "step_s": [round(s, 3) for s in tim["step_s"]]}
+            "cpu": cpu,
+            "step_s": [round(x, 3) for x in tim["step_s"]],
+            "config1_full": {"steps": n1, "wall_s": round(wall1, 3), "setup_s": round(tim1["setup_s"], 3),
+                             "value": f1.size * n1 / sum(tim1["step_s"]), "unit": "voxel-z-steps/s"}}
 
 
-def timed_march(field_np, params, dz, K, Wm, P, net=None, dpdt_max=0.0, tune=None):
+def timed_march(field_np, params, dz, K, Wm, P, net=None, dpdt_max=0.0, tune=None, launch=None):
     """W warm-up steps, K timed steps (synchronised on both sides), then P steps with
     HIP events around every launch.  Returns (seconds, per-kernel dict, max |p| seen)."""
     import ctypes
@@ -77,7 +104,9 @@ def timed_march(field_np, params, dz, K, Wm, P, net=None, dpdt_max=0.0, tune=Non
     nX, nY, nT = field_np.shape
     plan = MarchPlan(field_np.shape, params)
     plan.set_dz(dz)
-    if tune is not None:
+    if launch is not None:
+        plan.set_launch(*launch)
+    elif tune is not None:
         # the launch configuration (radix orders, column grid) chosen by the autotuner
         tune.update(plan.autotune())
     if net is not None:
@@ -182,27 +211,37 @@ def learned_leg(field_np, kw, dz, K, Wm, P, plain_ms, n_pairs, epochs):
                    "fp32 accumulate; trained on the GPU by the library's own kernels"}
 
 
-def batch_leg(n, seed, prop):
-    """32 seeded transmit configurations of uneven size solved as one batch (autotuned
-    launches), against the same instances solved one by one."""
+def batch_leg(n, seed, prop, sizes=(128, 192, 256, 384), lengths=(512, 1024, 2048), branches=(1, 4)):
+    """Seeded transmit configurations of uneven size solved as one batch (autotuned
+    launches), with the instances in 1 and in 4 concurrent graph branches, against the
+    same instances solved one by one."""
     import numpy as np
     import torch
     from angularspectrum_heterogeneous_amd import workloads as W
     from angularspectrum_heterogeneous_amd.batch import BatchSolve
     from angularspectrum_heterogeneous_amd.solver import SolverParams, angular_spectrum_solve
-    spec = W.config4(n=n, seed=seed, propDist=prop)
+    spec = W.config4(n=n, seed=seed, propDist=prop, sizes=sizes, lengths=lengths)
     inst = [(f, SolverParams(**kw)) for f, kw, _ in spec]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    b = BatchSolve(inst, autotune=True)
-    t_setup = time.perf_counter() - t0
-    tuned = [dict(it.plan.tuned) for it in b.items]
-    t0 = time.perf_counter()
-    res = b.run()
-    torch.cuda.synchronize()
-    t_batch = time.perf_counter() - t0
-    b.close()
-    work = sum(int(np.prod(f.shape)) * len(r[5]) for (f, _), r in zip(inst, res))
+    runs = {}
+    res = None
+    tuned = None
+    for nb in branches:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b = BatchSolve(inst, autotune=True, branches=nb)
+        t_setup = time.perf_counter() - t0
+        if tuned is None:
+            tuned = [dict(it.plan.tuned) for it in b.items]
+        t0 = time.perf_counter()
+        r = b.run()
+        torch.cuda.synchronize()
+        runs[nb] = {"batch_s": round(time.perf_counter() - t0, 3), "setup_and_autotune_s": round(t_setup, 3)}
+        b.close()
+        if res is None:
+            res = r
+        else:
+            del r
+    work = sum(int(np.prod(f.shape)) * len(r_[5]) for (f, _), r_ in zip(inst, res))
     errs = []
     t_single = 0.0
     for (f, p), r in zip(inst, res):
@@ -223,10 +262,14 @@ def batch_leg(n, seed, prop):
     for f, _, m in spec:
         key = f"{m['nX']}x{m['nX']}x{m['nT']}"
         shapes[key] = shapes.get(key, 0) + 1
+    for nb, rr in runs.items():
+        rr["value"] = work / rr["batch_s"]
+    best = min(runs, key=lambda k: runs[k]["batch_s"])
     return {"instances": n, "seed": seed, "propDist_m": prop, "shapes": shapes,
-            "voxel_z_steps": work, "batch_s": round(t_batch, 3), "setup_and_autotune_s": round(t_setup, 3),
-            "value": work / t_batch, "unit": "voxel-z-steps/s",
+            "voxel_z_steps": work, "batch_s": runs[best]["batch_s"], "branches": best,
+            "value": runs[best]["value"], "unit": "voxel-z-steps/s", "by_branches": runs,
             "one_by_one_s": round(t_single, 3), "one_by_one_value": work / t_single,
+            "speedup_vs_one_by_one": t_single / runs[best]["batch_s"],
             "max_rel_l2_vs_single": worst,
             "autotune_choices": {k: sorted({str((t["x_radix_order"], t["y_radix_order"], t["column_grid"]))
                                             for t, (_, _, m) in zip(tuned, spec)
@@ -241,7 +284,7 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--profile-iters", type=int, default=5)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned nonlinear step leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-solve leg")
@@ -250,6 +293,9 @@ def main():
     ap.add_argument("--batch-instances", type=int, default=32)
     ap.add_argument("--learned-pairs", type=int, default=200_000)
     ap.add_argument("--learned-epochs", type=int, default=3)
+    ap.add_argument("--launch", type=str, default=None,
+                    help="pin the launch configuration 'x_radix,y_radix,column_grid' instead of autotuning "
+                         "(a profile then holds only the march's own dispatches)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run 3 elementwise passes over 512 MiB (known HBM bytes) to scale PMC byte counters")
     args = ap.parse_args()
@@ -269,26 +315,39 @@ def main():
     vox = nX * nY * nT
     dz = params.dZmin
     K, Wm, P = args.steps, args.warmup, args.profile_iters
-    tune = None if args.no_autotune else {}
-    elapsed, kernels, fin = timed_march(field_np, params, dz, K, Wm, P, tune=tune)
+    launch = tuple(int(v) for v in args.launch.split(",")) if args.launch else None
+    tune = None if (args.no_autotune or launch) else {}
+    elapsed, kernels, fin = timed_march(field_np, params, dz, K, Wm, P, tune=tune, launch=launch)
+    if launch:
+        tune = {"x_radix_order": launch[0], "y_radix_order": launch[1], "column_grid": launch[2]}
     value = vox * K / elapsed
     ms_step = elapsed / K * 1e3
+    # rows of every x row the super-absorbing y pass touches: the two boundary bands
+    # (rows 0..b and nY-1-b..nY-1, b = round(nY boundaryFactor))
+    band = max(round(nY * params.boundaryFactor), 1)
+    band_rows = min(2 * (band + 1), nY)
     dom = max(kernels, key=lambda n: kernels[n]["avg_ms"] * kernels[n]["launches_per_step"])
     dom_s = kernels[dom]["avg_ms"] * 1e-3
-    ab = algorithmic_bytes_per_launch(dom, vox, nX * nY, nT)
+    ab = algorithmic_bytes_per_launch(dom, vox, nX * nY, nT, nX, band_rows)
     achieved = ab / dom_s / 1e9
+    step_bytes = 0.0
     for name, kv in kernels.items():
-        b = algorithmic_bytes_per_launch(name, vox, nX * nY, nT)
+        b = algorithmic_bytes_per_launch(name, vox, nX * nY, nT, nX, band_rows)
+        kv["alg_bytes_per_launch"] = b
         kv["alg_GBps"] = round(b / (kv["avg_ms"] * 1e-3) / 1e9, 1)
+        step_bytes += b * kv["launches_per_step"]
     # measured HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated
     # on a kernel of known traffic), recorded next to the profile summaries
     traffic = None
+    measured_step_bytes = None
     tp = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic_256x256x1024.json")
     if os.path.exists(tp):
-        t = json.load(open(tp))["kernels"].get(dom)
+        tk = json.load(open(tp))["kernels"]
+        t = tk.get(dom)
         if t:
             traffic = t["traffic_bytes"]
-    step_bytes = 104.0 * vox  # 13 sweeps x 8 B at this configuration
+        if all(n in tk for n in kernels):
+            measured_step_bytes = sum(tk[n]["traffic_bytes"] * kv["launches_per_step"] for n, kv in kernels.items())
     step_frac = step_bytes / (elapsed / K) / 1e9 / HBM_PEAK_GBS
 
     dev = torch.device("cuda")
@@ -334,13 +393,19 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "alg_bytes_per_launch": ab},
-        "step_roofline": {"alg_bytes_per_voxel_step": 104, "frac": round(step_frac, 4)},
+        "step_roofline": {"alg_bytes_per_voxel_step": round(step_bytes / vox, 2), "frac": round(step_frac, 4),
+                          "measured_bytes_per_voxel_step": (round(measured_step_bytes / vox, 2)
+                                                            if measured_step_bytes else None),
+                          "measured_frac": (round(measured_step_bytes / (elapsed / K) / 1e9 / HBM_PEAK_GBS, 4)
+                                            if measured_step_bytes else None)},
         "stream_reference": {"kernel": "elementwise add, 512 MiB in + 512 MiB out", "GBps": round(stream_gbs, 1),
                              "frac_of_peak": round(stream_gbs / HBM_PEAK_GBS, 4)},
         "kernels": kernels,
     }
     if not args.no_batch:
         result["batch"] = batch_leg(args.batch_instances, 0, 0.02)
+        # the small grids, where one instance leaves most of the GPU idle
+        result["batch_small"] = batch_leg(16, 1, 0.02, sizes=(128, 192), lengths=(512, 1024))
     if not args.no_learned:
         result["learned"] = learned_leg(field_np, kw, dz, K, Wm, P, ms_step, args.learned_pairs,
                                         args.learned_epochs)

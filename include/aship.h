@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define ASH_ABI_VERSION 2
+#define ASH_ABI_VERSION 3
 
 #define ASH_OK 0
 #define ASH_E_ARG (-1)         /* invalid argument */
@@ -48,9 +48,20 @@ extern "C" {
 #define ASH_NL_RUSANOV 2     /* _rusanov_flux_standard, ASS:688-696 */
 #define ASH_NL_TVD 3         /* _rusanov_flux_tvd, ASS:699-724 */
 #define ASH_NL_KT_MINMOD 4   /* _kt_flux_minmod, ASS:862-868 */
-/* sub-cycling of the KT update (0 = none, as the reference march) */
+/* CFL sub-cycling of the KT update (0 = none, as the reference march).  A step
+ * whose CFL number N max|u| dz / dT exceeds cflTarget is split into
+ * ceil(CFL / cflTarget) SSP-RK2 sub-steps of dz / n.
+ *   GLOBAL: one count for the whole field from its max|u|, as _kt_flux_adaptive
+ *           (ASS:842-859), with the maximum reduced on the device (no host sync).
+ *   COLUMN: each (x, y) column from its own max|u| (extension; equal to GLOBAL
+ *           wherever the field's CFL number is at or below the target).
+ * Sub-cycling applies to the KT schemes only.  The kernels take at most 1024
+ * sub-steps per dz; a column (or field) that would need more is counted, see
+ * ash_plan_subcycle_capped. */
 #define ASH_SUBCYCLE_OFF 0
 #define ASH_SUBCYCLE_COLUMN 1
+#define ASH_SUBCYCLE_GLOBAL 2
+#define ASH_MAX_SUBSTEPS 1024
 
 /* Solver description: the fields of SolverParams (ASS:35-76) that shape the
  * march, plus the grid.  Flags are 0/1. */
@@ -74,7 +85,7 @@ typedef struct ash_desc {
   double superAbsorbingStrength;
   int32_t fluxScheme;              /* 0 rusanov, 1 kt (ASS:64) */
   int32_t useObliquityCorrection;
-  int32_t subcycle;                /* ASH_SUBCYCLE_* (extension, default off) */
+  int32_t subcycle;                /* ASH_SUBCYCLE_* (extension, default off; split-step KT only) */
   double cflTarget;                /* target CFL of sub-cycling (0.5 in ASS:842) */
   int32_t useNonlinearityObliquity; /* N scaled by the beam's power-weighted <k/k_z> (ASS:923-965) */
 } ash_desc;
@@ -107,6 +118,9 @@ typedef struct ash_step_io {
   const double* src_delay_dev;     /* [nX*nY] */
   const double* taxis_dev;         /* [nT] */
   double src_f0, src_p0, src_ncycles, src_dur;
+  /* time of flight after the step (ash_plan_set_tof): retarded arrival time of each
+   * column, (max_rows, nX, nY); NULL = not computed */
+  float* tof_dev;
 } ash_step_io;
 
 /* ABI version (ASH_ABI_VERSION). */
@@ -114,6 +128,11 @@ int ash_abi_version(void);
 
 /* Text of the last error raised on the calling thread. */
 const char* ash_last_error(void);
+
+/* The (n-1)-normalised, mean-subtracted wavenumber grid of ASS:403-408
+ * (2 pi j / ((n-1) d) minus its NumPy mean) in natural FFT order, as a plan's
+ * tables hold it; out is host memory of n doubles.  Host only, no device use. */
+int ash_centred_wavenumbers(int32_t n, double d, double* out);
 
 /* Creates a plan; replaces the operator precompute of ASS:1301-1371
  * (_build_operators / _to_jax).  Operators are evaluated on the fly, so the
@@ -158,14 +177,24 @@ int ash_attenuation_step(ash_plan* plan, const float* in_dev, float* out_dev, do
 int ash_angular_spectrum_step(ash_plan* plan, const float* in_dev, float* out_dev, int32_t half, void* stream);
 
 /* _kt_flux, _kt_flux_minmod, _rusanov_flux_standard, _rusanov_flux_tvd
- * (ASS:688-868) on every (x, y) column, with step dz; subcycle =
- * ASH_SUBCYCLE_COLUMN gives per-column CFL sub-cycling (cflTarget). */
+ * (ASS:688-868) on every (x, y) column, with step dz.  subcycle =
+ * ASH_SUBCYCLE_GLOBAL is _kt_flux_adaptive (ASS:842-859) at the plan's
+ * cflTarget; ASH_SUBCYCLE_COLUMN sub-cycles per column.  Sub-cycling with a
+ * non-KT scheme is ASH_E_ARG.  in_dev and out_dev may alias. */
 int ash_nonlinear_step(ash_plan* plan, const float* in_dev, float* out_dev, int32_t scheme, double dz,
                        double beta_tvd, int32_t subcycle, void* stream);
 
 /* Global maximum of |field| (the float(jnp.max(jnp.abs(field))) of ASS:849),
- * written as float bits into *out_bits_dev (must be zeroed by the caller). */
+ * written as float bits into *out_bits_dev.  The plan's field state is not
+ * touched. */
 int ash_max_abs(ash_plan* plan, const float* in_dev, uint32_t* out_bits_dev, void* stream);
+
+/* Columns (per-column mode) or fields (global mode) whose sub-cycling needed
+ * more than ASH_MAX_SUBSTEPS sub-steps since the last ash_load_field (or plan
+ * creation), counted on the device; they took ASH_MAX_SUBSTEPS (one step for a
+ * NaN amplitude).  The reference has no bound (its loop runs on the host).
+ * Synchronises with the device. */
+int ash_plan_subcycle_capped(ash_plan* plan, int64_t* out);
 
 /* _freq_weighted_boundary_step (ASS:658-664) with precalculate_abl_freq. */
 int ash_freq_weighted_boundary_step(ash_plan* plan, const float* in_dev, float* out_dev, void* stream);
@@ -223,6 +252,14 @@ typedef struct ash_batch ash_batch;
 int ash_batch_create(ash_batch** out);
 void ash_batch_destroy(ash_batch* batch);
 int ash_batch_step(ash_batch* batch, ash_plan* const* plans, const ash_step_io* ios, int32_t n, void* stream);
+
+/* Concurrent branches of the batch's captured step (1 = one serial chain; default
+ * and maximum ASH_BATCH_MAX_BRANCHES, the hardware queues of a process): the
+ * instances are spread over the branches by voxel count, largest first, and the
+ * branches fork from and join into the launch stream, so small grids of
+ * different instances overlap on the GPU.  The results do not depend on it. */
+#define ASH_BATCH_MAX_BRANCHES 4
+int ash_batch_set_branches(ash_batch* batch, int32_t n);
 
 /* Timing aid with no counterpart in the reference (whose only timing is the
  * wall clock of ASS:1411, :1552): runs n_iter steps like ash_step, without the
@@ -285,6 +322,25 @@ int ash_net_adam(ash_net* net, const float* grads_dev, float lr, float beta1, fl
  * left unchanged. */
 int ash_net_apply(ash_net* net, float* u_dev, int32_t ncols, double c, double dpdt_max, double dT,
                   int32_t* flag_dev, void* stream);
+
+/* Time of flight per (x, y) column after every step (ash_step_io.tof_dev),
+ * replacing the reference's calls of its tof_extraction module (imported at
+ * ASS:26-29, called at ASS:1538-1545; the module is not shipped and its
+ * estimators are documented only at ASS:1244-1264).  The value written is the
+ * retarded arrival time; the caller adds z / c0 for the arrival time at depth z.
+ *   ASH_TOF_ENVELOPE: the first time the Hilbert envelope of the column reaches
+ *     env_ratio of its maximum, interpolated linearly from the sample before
+ *     (env_ratio = 1: the envelope-peak sample);
+ *   ASH_TOF_MATCHED: t_ref_peak plus the lag of the first maximum of the
+ *     circular cross-correlation with ref_dev (nT samples, device memory),
+ *     refined by parabolic interpolation over the peak and its two neighbours.
+ * Sample k is at time t0 + k dt.  A column that is zero everywhere gets NaN.
+ * ASH_TOF_OFF disables the pass. */
+#define ASH_TOF_OFF 0
+#define ASH_TOF_ENVELOPE 1
+#define ASH_TOF_MATCHED 2
+int ash_plan_set_tof(ash_plan* plan, int32_t mode, double env_ratio, double t0, double dt, const float* ref_dev,
+                     double t_ref_peak, void* stream);
 
 /* Use the network for the nonlinear update of the plan's steps (null: the KT
  * update).  Split-step KT plans only; nT must match.  The plan keeps a real

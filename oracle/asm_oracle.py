@@ -257,13 +257,16 @@ def apply_phase_screen(field, phase_screen, f0_bin, amplitude_screen=None):
 # ---------------------------------------------------------------------------
 # Wavenumber grids (ASS:403-408, 488-491)
 # ---------------------------------------------------------------------------
-def centered_grid(n: int, d: float) -> np.ndarray:
-    """(n-1)-normalised, mean-subtracted wavenumber grid (ASS:403-408).
+def centered_grid(n: int, d: float, c0: float = None) -> np.ndarray:
+    """(n-1)-normalised, mean-subtracted wavenumber grid (ASS:403-408); with ``c0``
+    the ASM t grid, divided by c0 before the mean is taken, as ASS:403-404 does.
 
     Index j of the result multiplies the fftshifted bin j, i.e. natural FFT
     bin ``(j - n//2) % n``.
     """
     k = np.linspace(0, n - 1, n) / (n - 1) / d * 2 * np.pi
+    if c0 is not None:
+        k = k / c0
     return k - np.mean(k)
 
 
@@ -300,7 +303,7 @@ def precalculate_mas(nX, nY, nT, dX, dY, dZ, dT, c0, split_step=False,
     HH when only the split-step half operator is needed (the reference builds
     both; the unused one does not change any result).
     """
-    kt = centered_grid(nT, dT) / c0
+    kt = centered_grid(nT, dT, c0)
     kx = centered_grid(nX, dX)
     ky = centered_grid(nY, dY)
     kk = kx[:, None] ** 2 + ky[None, :] ** 2
@@ -865,12 +868,16 @@ def march_step(field, ops, p: OracleParams, N, dZ, kt_subcycle=None, obl_map=Non
 def angular_spectrum_solve(initial_field, p: OracleParams, dtype=np.float64,
                            per_step_callback: Optional[Callable] = None,
                            max_steps_run: Optional[int] = None, kt_subcycle=None,
-                           timings: Optional[dict] = None):
-    """ASS:1217-1567 restated (TOF outputs excluded: the module is not shipped).
+                           timings: Optional[dict] = None, tof_ref_trace=None, tof_t_ref_peak_s=None,
+                           tof_env_ratio=None, taxis=None):
+    """ASS:1217-1567 restated.
 
     ``max_steps_run`` stops after that many steps (used to time a bounded
     sample); ``timings`` receives 'setup_s' and per-step wall times 'step_s'.
-    Returns (field, pnp, ppp, pI, pIloss, zaxis, pax) like ASS:1567.
+    Returns (field, pnp, ppp, pI, pIloss, zaxis, pax) like ASS:1567, with a
+    per-step TOF map appended (ASS:1387-1397, :1538-1545, :1563-1566) when
+    ``tof_ref_trace`` or ``tof_env_ratio`` is given; the TOF estimators are
+    ``tof_oracle``'s restatement of the absent ``tof_extraction`` module.
     """
     import time
     t_setup = time.perf_counter()
@@ -893,6 +900,12 @@ def angular_spectrum_solve(initial_field, p: OracleParams, dtype=np.float64,
     f0_bin = p.f0 / (1.0 / (nT * p.dT))
     obl_map = (precalculate_obliquity_map(nX, nY, nT, p.dX, p.dY, p.dT, c0)
                if p.useNonlinearityObliquity else None)
+    tof_mf = tof_ref_trace is not None
+    tof_on = tof_mf or tof_env_ratio is not None
+    if tof_mf and tof_t_ref_peak_s is None:
+        raise ValueError('tof_t_ref_peak_s is required when tof_ref_trace is provided')
+    tof_t = np.asarray(taxis, dtype=np.float64) if taxis is not None else np.arange(nT, dtype=np.float64) * p.dT
+    from oracle import tof_oracle
 
     def alloc(ms):
         return (np.zeros((nX, nY, ms), np.float32), np.zeros((nX, nY, ms), np.float32),
@@ -901,6 +914,7 @@ def angular_spectrum_solve(initial_field, p: OracleParams, dtype=np.float64,
 
     max_steps = int(np.ceil(p.propDist / dZ)) + 100
     pnp, ppp, pI, pIloss, pax = alloc(max_steps)
+    tof = np.zeros((nX, nY, max_steps), np.float32) if tof_on else None
     field = init.astype(dtype)
     zvec = []
     cc = 0
@@ -925,6 +939,7 @@ def angular_spectrum_solve(initial_field, p: OracleParams, dtype=np.float64,
             ops = build_operators(nX, nY, nT, p, dZ, dtype)
             max_steps = int(np.ceil(p.propDist / dZ)) + 100
             pnp, ppp, pI, pIloss, pax = alloc(max_steps)
+            tof = np.zeros((nX, nY, max_steps), np.float32) if tof_on else None
             continue
         I_before = np.sum(field.astype(np.float64) ** 2, axis=2)
         step_ops = ops
@@ -952,11 +967,17 @@ def angular_spectrum_solve(initial_field, p: OracleParams, dtype=np.float64,
         ppp[:, :, cc] = np.max(field, axis=2)
         pax[:, cc] = field[nX // 2, nY // 2, :]
         zvec.append(dZ_step)
+        if tof_on:
+            if tof_mf:
+                tof[:, :, cc] = tof_oracle.extract_tof_matched_filter_parabolic(
+                    field, tof_t, sum(zvec), c0, tof_ref_trace, float(tof_t_ref_peak_s))
+            else:
+                tof[:, :, cc] = tof_oracle.extract_tof_envelope(field, tof_t, sum(zvec), c0, tof_env_ratio)
         if per_step_callback is not None:
             per_step_callback(cc, sum(zvec), field)
         cc += 1
         if timings is not None:
             timings['step_s'].append(time.perf_counter() - t_step)
     zaxis = np.cumsum(zvec[:cc])
-    return (field, pnp[:, :, :cc], ppp[:, :, :cc], pI[:, :, :cc], pIloss[:, :, :cc],
-            zaxis, pax[:, :cc])
+    out = (field, pnp[:, :, :cc], ppp[:, :, :cc], pI[:, :, :cc], pIloss[:, :, :cc], zaxis, pax[:, :cc])
+    return out + (tof[:, :, :cc],) if tof_on else out

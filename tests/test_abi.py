@@ -78,10 +78,10 @@ def test_plan_create_rejects_unsupported_sizes_before_device_use():
     lib = _lib.load()
     p = SolverParams(dX=3e-4, dY=3e-4, dT=4e-8)
     h = ctypes.c_void_p()
-    for shape in [(51, 51, 101), (64, 64, 300), (100, 64, 256)]:
+    for shape in [(1, 51, 101), (64, 1025, 256), (64, 64, 3), (64, 64, 4096)]:
         d = desc_from_params(shape, p)
         assert lib.ash_plan_create(ctypes.byref(d), ctypes.byref(h)) == -2
-        assert b"must be" in lib.ash_last_error()
+        assert b"must be between" in lib.ash_last_error()
 
 
 def test_error_codes_match_header():
@@ -110,12 +110,33 @@ def test_create_failing_on_the_device_returns_an_error_and_no_handle():
     n = ctypes.c_void_p(1)
     assert lib.ash_net_create(256, 16, ctypes.byref(n)) == _lib.ASH_E_HIP
     assert n.value is None and b"hipMalloc" in lib.ash_last_error()
+
+
+def test_argument_errors_are_reported_before_any_device_use():
+    """Invalid arguments return ASH_E_ARG (or ASH_E_UNSUPPORTED) with a message,
+    with or without a GPU: the checks run before any allocation or launch."""
+    from angularspectrum_heterogeneous_amd import _lib
+    from angularspectrum_heterogeneous_amd.plan import desc_from_params
+    from angularspectrum_heterogeneous_amd.solver import SolverParams
+    lib = _lib.load()
+    h = ctypes.c_void_p()
     d = desc_from_params((64, 64, 256), SolverParams(dX=0.0, dY=3e-4, dT=4e-8))
-    assert lib.ash_plan_create(ctypes.byref(d), ctypes.byref(h)) == -1
-    assert lib.ash_plan_create(None, ctypes.byref(h)) == -1
-    assert lib.ash_source_field(None, None, None, 4, 8, 1e6, 1.0, 3.0, 2.0, None, None) == -1
+    assert lib.ash_plan_create(ctypes.byref(d), ctypes.byref(h)) == _lib.ASH_E_ARG
+    assert lib.ash_plan_create(None, ctypes.byref(h)) == _lib.ASH_E_ARG
+    assert lib.ash_source_field(None, None, None, 4, 8, 1e6, 1.0, 3.0, 2.0, None, None) == _lib.ASH_E_ARG
     assert b"null" in lib.ash_last_error()
-    assert lib.ash_abi_version() == 2
+    # CFL sub-cycling: KT only, known modes only
+    d = desc_from_params((64, 64, 256), SolverParams(dX=3e-4, dY=3e-4, dT=4e-8, fluxScheme='kt'), "global")
+    d.fluxScheme = 0
+    assert lib.ash_plan_create(ctypes.byref(d), ctypes.byref(h)) == _lib.ASH_E_ARG
+    assert b"KT" in lib.ash_last_error()
+    d.fluxScheme, d.subcycle = 1, 7
+    assert lib.ash_plan_create(ctypes.byref(d), ctypes.byref(h)) == _lib.ASH_E_ARG
+    with pytest.raises(ValueError, match="KT"):
+        desc_from_params((64, 64, 256), SolverParams(dX=3e-4, dY=3e-4, dT=4e-8), "column")
+    with pytest.raises(ValueError, match="ash_subcycle"):
+        desc_from_params((64, 64, 256), SolverParams(dX=3e-4, dY=3e-4, dT=4e-8, fluxScheme='kt'), "always")
+    assert lib.ash_abi_version() == _lib.ABI_VERSION == 3
 
 
 def test_solver_params_match_reference_fields_and_defaults():
@@ -173,3 +194,21 @@ def test_bench_refuses_more_than_one_gpu():
                        text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU only" in r.stderr
+
+
+def test_wavenumber_grids_equal_numpys_bit_for_bit():
+    """The plan's kx/ky tables (host FP64) equal the reference's NumPy grids
+    (ASS:405-408: linspace / (n-1) / d * 2 pi minus np.mean) to the last bit, in
+    natural FFT order; a grid value one ulp off can flip a mode at the evanescent
+    cutoff (k_z = 0) of the attenuation's obliquity factor."""
+    import numpy as np
+    from angularspectrum_heterogeneous_amd import _lib
+    lib = _lib.load()
+    for d in (3.08e-4, 1e-4, 0.1 * 1.5e-3 / 3, 1.0):
+        for n in list(range(2, 140)) + [192, 255, 256, 257, 365, 384, 511, 601, 1000, 1024]:
+            k = np.linspace(0, n - 1, n) / (n - 1) / d * 2 * np.pi
+            k -= np.mean(k)
+            want = k[(np.arange(n) + n // 2) % n]          # natural bin i <- fftshifted index (i + n//2) % n
+            got = (ctypes.c_double * n)()
+            assert lib.ash_centred_wavenumbers(n, d, got) == 0
+            np.testing.assert_array_equal(np.frombuffer(got, dtype=np.float64), want, err_msg=f"n={n} d={d}")

@@ -3,7 +3,10 @@
 Each entry point of the C ABI is called on a seeded field and compared with
 the oracle's restatement of the reference function it replaces.  The bound is
 relative L2 error; single-precision kernels on one operator land at ~1e-7, the
-tolerance is 2e-6 (5e-6 for the multi-sweep composite step).
+tolerance is 2e-6 (5e-6 for the multi-sweep composite step).  GEN sizes run the
+general-size kernels on one, two or all three axes (odd, prime and 2^a 3^b 5^c
+lengths, odd nT without a Nyquist bin); their direct DFTs accumulate n terms per
+output in float32, so their bounds are 5e-6 (1e-5 composite).
 """
 import numpy as np
 import pytest
@@ -14,6 +17,12 @@ from tests.conftest import rel_l2
 pytestmark = pytest.mark.gpu
 
 SIZES = [(64, 64, 256), (128, 192, 512), (256, 256, 1024), (384, 64, 2048), (64, 384, 512)]
+GEN = [(51, 51, 101), (32, 32, 120), (64, 51, 256), (51, 64, 300), (64, 64, 101), (96, 81, 480)]
+
+
+def _tol(shape, fast, general):
+    from angularspectrum_heterogeneous_amd._lib import load  # noqa: F401  (the library must load)
+    return general if shape in GEN else fast
 
 
 def _params(shape, **over):
@@ -46,7 +55,7 @@ def _dev(a):
     return torch.from_numpy(np.ascontiguousarray(a)).cuda()
 
 
-@pytest.mark.parametrize("shape", SIZES[:2])
+@pytest.mark.parametrize("shape", SIZES[:2] + GEN[:2] + GEN[4:5])
 @pytest.mark.parametrize("scheme", ["kt", "kt_minmod", "rusanov", "tvd"])
 def test_nonlinear_step(shape, scheme):
     from angularspectrum_heterogeneous_amd import _lib
@@ -68,7 +77,7 @@ def test_nonlinear_step(shape, scheme):
     assert rel_l2(got - u, ref - u64) < 1e-4  # the increment itself
 
 
-@pytest.mark.parametrize("shape", SIZES)
+@pytest.mark.parametrize("shape", SIZES + GEN)
 def test_attenuation_step(shape):
     p = _params(shape)
     nX, nY, nT = shape
@@ -81,10 +90,10 @@ def test_attenuation_step(shape):
     ref = O.attenuation_step(u.astype(np.float64), afh)
     e = rel_l2(got, ref)
     print(shape, e)
-    assert e < 2e-6
+    assert e < _tol(shape, 2e-6, 5e-6)
 
 
-@pytest.mark.parametrize("shape", SIZES)
+@pytest.mark.parametrize("shape", SIZES + GEN)
 @pytest.mark.parametrize("half", [True, False])
 def test_angular_spectrum_step(shape, half):
     p = _params(shape)
@@ -99,10 +108,10 @@ def test_angular_spectrum_step(shape, half):
     ref = O.angular_spectrum_step(u.astype(np.float64), HHh if half else HH, ablh if half else abl)
     e = rel_l2(got, ref)
     print(shape, half, e)
-    assert e < 2e-6
+    assert e < _tol(shape, 2e-6, 5e-6)
 
 
-@pytest.mark.parametrize("shape", SIZES[:3])
+@pytest.mark.parametrize("shape", SIZES[:3] + GEN)
 def test_march_step_split_kt(shape):
     p = _params(shape)
     nX, nY, nT = shape
@@ -115,10 +124,10 @@ def test_march_step_split_kt(shape):
     ref = O.march_step(u.astype(np.float64), ops, p, N, dz)
     e = rel_l2(got, ref)
     print(shape, e)
-    assert e < 5e-6
+    assert e < _tol(shape, 5e-6, 1e-5)
 
 
-@pytest.mark.parametrize("shape", SIZES[:3] + SIZES[4:])
+@pytest.mark.parametrize("shape", SIZES[:3] + SIZES[4:] + GEN[:1] + GEN[3:4])
 def test_march_step_nonlinearity_obliquity(shape):
     """The composite step with N scaled by the beam obliquity (ASS:956-965)."""
     p = _params(shape, useNonlinearityObliquity=True)
@@ -133,7 +142,7 @@ def test_march_step_nonlinearity_obliquity(shape):
     ref = O.march_step(u.astype(np.float64), ops, p, N, dz, obl_map=obl)
     e = rel_l2(got, ref)
     print(shape, e)
-    assert e < 5e-6
+    assert e < _tol(shape, 5e-6, 1e-5)
 
 
 @pytest.mark.parametrize("shape", SIZES)
@@ -155,7 +164,7 @@ def test_march_step_launch_configs(shape, launch):
     assert e < 5e-6
 
 
-@pytest.mark.parametrize("shape", SIZES[:2] + SIZES[3:4])
+@pytest.mark.parametrize("shape", SIZES[:2] + SIZES[3:4] + GEN)
 def test_freq_weighted_boundary(shape):
     p = _params(shape, useFreqWeightedBoundary=True)
     nX, nY, nT = shape
@@ -166,10 +175,10 @@ def test_freq_weighted_boundary(shape):
     ref = O.freq_weighted_boundary_step(u.astype(np.float64), af)
     e = rel_l2(got, ref)
     print(shape, e)
-    assert e < 2e-6
+    assert e < _tol(shape, 2e-6, 5e-6)
 
 
-@pytest.mark.parametrize("shape", SIZES[:2] + SIZES[3:4])
+@pytest.mark.parametrize("shape", SIZES[:2] + SIZES[3:4] + GEN)
 def test_super_absorbing(shape):
     p = _params(shape, useSuperAbsorbing=True)
     nX, nY, nT = shape
@@ -182,10 +191,10 @@ def test_super_absorbing(shape):
                                  p.dX, p.dY, p.dT, p.c0, p.superAbsorbingStrength)
     e = rel_l2(got, ref)
     print(shape, e)
-    assert e < 2e-6
+    assert e < _tol(shape, 2e-6, 5e-6)
 
 
-@pytest.mark.parametrize("shape", SIZES[:2])
+@pytest.mark.parametrize("shape", SIZES[:2] + GEN[:2] + GEN[3:4])
 @pytest.mark.parametrize("with_amp", [False, True])
 def test_phase_screen(shape, with_amp):
     p = _params(shape)

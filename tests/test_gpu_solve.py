@@ -123,13 +123,24 @@ def test_callback_sees_each_step():
 
 
 def test_tof_arguments_rejected():
+    """A reference trace needs its peak time (ASS:1392-1395); the trace and the
+    time axis must have nT samples and the axis must be uniform."""
     from angularspectrum_heterogeneous_amd import workloads as W
     from angularspectrum_heterogeneous_amd.solver import SolverParams, angular_spectrum_solve
     field, kw = W.config1()
-    with pytest.raises(ValueError):
-        angular_spectrum_solve(field, SolverParams(**kw), verbose=False, tof_ref_trace=np.zeros(256))
-    with pytest.raises(NotImplementedError):
-        angular_spectrum_solve(field, SolverParams(**kw), verbose=False, tof_env_ratio=0.5)
+    nT = field.shape[2]
+    with pytest.raises(ValueError, match="tof_t_ref_peak_s"):
+        angular_spectrum_solve(field, SolverParams(**kw), verbose=False, tof_ref_trace=np.zeros(nT))
+    with pytest.raises(ValueError, match="tof_ref_trace"):
+        angular_spectrum_solve(field, SolverParams(**kw), verbose=False, tof_ref_trace=np.zeros(nT - 1),
+                               tof_t_ref_peak_s=1e-6)
+    with pytest.raises(ValueError, match="taxis"):
+        angular_spectrum_solve(field, SolverParams(**kw), verbose=False, tof_env_ratio=0.5,
+                               taxis=np.arange(nT + 1) * kw["dT"])
+    warped = np.arange(nT) * kw["dT"]
+    warped[nT // 2:] += 0.5 * kw["dT"]
+    with pytest.raises(ValueError, match="uniform"):
+        angular_spectrum_solve(field, SolverParams(**kw), verbose=False, tof_env_ratio=0.5, taxis=warped)
 
 
 def _focused_small(p0=1e6):

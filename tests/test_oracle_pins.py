@@ -174,6 +174,26 @@ def test_attenuation_filter_unity_at_dc_and_evanescent():
     np.testing.assert_allclose(afh ** 2, af, rtol=1e-12)
 
 
+def test_cutoff_mode_depends_on_unfused_rounding_of_kx2_plus_ky2():
+    """On the 96 x 81 x 480 grid one mode (m = 60, shifted index (0, 10)) sits
+    within an ulp of the evanescent cutoff: kx^2 + ky^2 rounded after each
+    operation (NumPy, ASS:492) and the same sum through one fused multiply-add
+    fall on opposite sides of k_m^2, and the attenuation factor there is 1 on one
+    side and exp(-alpha dz 1e6) ~ 0 on the other.  The GPU parity case of that
+    shape therefore fails unless the kernels evaluate kk without contraction."""
+    from fractions import Fraction
+    nX, nY, nT, d, dT, c0 = 96, 81, 480, 3.08e-4, 4e-8, 1540.0
+    kx, ky = O.centered_grid(nX, d), O.centered_grid(nY, d)
+    m, i, j = 60, 0, 10
+    km = (2 * np.pi * (np.arange(nT) / (nT * dT)) / c0)[m]
+    kk = kx[i] ** 2 + ky[j] ** 2
+    kk_fused = float(Fraction(ky[j]) * Fraction(ky[j]) + Fraction(kx[i] * kx[i]))  # fma(ky, ky, kx*kx)
+    assert (km ** 2 - kk > 0) != (km ** 2 - kk_fused > 0)
+    af = O.precalculate_ad_pow2(0.5, nX, nY, nT, d, d, d, dT, c0, 1e6, 1.1, n_bins=m + 1)[0]
+    assert (abs(af[i, j, m]) < 1e-30) == (km ** 2 - kk > 0)
+    assert (abs(af[i, j, m] - 1) == 0) == (km ** 2 - kk <= 0)
+
+
 def test_powerlaw_attenuation_and_kk_dispersion_vs_theory():
     """validate_solver.py:795-960: plane-wave run, measured alpha(f) within
     1e-3 dB/cm of alpha0 f^y and phase velocity matching the KK curve, 1-5 MHz."""
