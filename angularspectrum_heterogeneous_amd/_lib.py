@@ -77,6 +77,8 @@ _PROTOS = {
     "ash_nonlinear_step": ([c_vp, c_vp, c_vp, c_i32, c_dbl, c_dbl, c_i32, c_vp], c_i32),
     "ash_max_abs": ([c_vp, c_vp, c_vp, c_vp], c_i32),
     "ash_plan_subcycle_capped": ([c_vp, ctypes.POINTER(ctypes.c_int64)], c_i32),
+    "ash_nonlinear_step_noalloc": ([ctypes.c_int64, c_i32, c_dbl, c_dbl, c_vp, c_vp, c_i32, c_dbl, c_dbl, c_i32,
+                                    c_dbl, c_vp, c_vp], c_i32),
     "ash_freq_weighted_boundary_step": ([c_vp, c_vp, c_vp, c_vp], c_i32),
     "ash_super_absorbing_step": ([c_vp, c_vp, c_vp, c_vp], c_i32),
     "ash_apply_phase_screen": ([c_vp, c_vp, c_vp, c_vp, c_vp, c_vp], c_i32),

@@ -22,6 +22,16 @@
 
 using namespace ash;
 
+// Scratch of ash_nonlinear_step_noalloc, the entry point that allocates nothing:
+// [0, NOALLOC_PARTIALS) per-block max|p| partials (one per 64 columns), then the
+// global maximum and the sub-cycling cap counter.  In host code `scratch` names
+// a host-side stand-in; kernels get its device address from hipGetSymbolAddress
+// (one address per device, looked up on the first call on that device).
+constexpr int NOALLOC_PARTIALS = 16384;  // columns: at most 64 * 16384 = 1,048,576
+constexpr int NOALLOC_GMAX = NOALLOC_PARTIALS;
+constexpr int NOALLOC_SUBCAP = NOALLOC_PARTIALS + 1;
+static __device__ __align__(256) unsigned scratch[NOALLOC_PARTIALS + 64];
+
 namespace {
 
 thread_local std::string g_err;
@@ -1398,6 +1408,110 @@ int ash_plan_subcycle_capped(ash_plan* p, int64_t* out) try {
   HIP_TRY(hipMemcpy(&v, p->red + RED_SUBCAP, sizeof(v), hipMemcpyDeviceToHost));
   *out = (int64_t)v;
   return ASH_OK;
+} catch (...) {
+  return caught_exception();
+}
+
+}  // extern "C"
+
+namespace {
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+unsigned* g_scratch_dev[NOALLOC_MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device, looked up once per device.
+int scratch_on_current_device(unsigned** out) {
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+    return fail(ASH_E_UNSUPPORTED, "device index beyond the no-allocation scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* ptr = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_dev[dev] = static_cast<unsigned*>(ptr);
+  }
+  *out = g_scratch_dev[dev];
+  return ASH_OK;
+}
+
+// k_column<M, COL_RAWNL, NLK> on ncols real columns without a plan (no tables:
+// the pass runs no transform).
+template <int M>
+void launch_rawnl_M(const ColDesc& cd, int ncols, hipStream_t s) {
+  Geom g{};
+  g.nX = ncols;
+  g.nY = 1;
+  g.nT = 2 * M;
+  g.M = M;
+  g.MP = 0;
+  const Tables T{};
+  const int per_block = ColCfg<M>::WPB * ColCfg<M>::CPW;
+  const int blocks = std::min((ncols + per_block - 1) / per_block, 2048);
+  switch (cd.nl) {
+    case NL_KT: hipLaunchKernelGGL((k_column<M, COL_RAWNL, NL_KT>), dim3(blocks), dim3(256), 0, s, nullptr, T, g, cd); break;
+    case NL_KT_MINMOD: hipLaunchKernelGGL((k_column<M, COL_RAWNL, NL_KT_MINMOD>), dim3(blocks), dim3(256), 0, s, nullptr, T, g, cd); break;
+    case NL_RUSANOV: hipLaunchKernelGGL((k_column<M, COL_RAWNL, NL_RUSANOV>), dim3(blocks), dim3(256), 0, s, nullptr, T, g, cd); break;
+    case NL_TVD: hipLaunchKernelGGL((k_column<M, COL_RAWNL, NL_TVD>), dim3(blocks), dim3(256), 0, s, nullptr, T, g, cd); break;
+    default: hipLaunchKernelGGL((k_column<M, COL_RAWNL, NL_NONE>), dim3(blocks), dim3(256), 0, s, nullptr, T, g, cd); break;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ash_nonlinear_step_noalloc(int64_t ncols, int32_t nT, double N, double dT, const float* in, float* out,
+                               int32_t scheme, double dz, double beta_tvd, int32_t subcycle, double cfl_target,
+                               uint32_t* report_dev, void* stream) try {
+  if (!in || !out) return fail(ASH_E_ARG, "null argument");
+  if (ncols < 1) return fail(ASH_E_ARG, "ncols must be positive");
+  if (ncols > (int64_t)NOALLOC_PARTIALS * ABSMAX_COLS)
+    return fail(ASH_E_UNSUPPORTED, "ncols exceeds the no-allocation scratch (at most 1048576 columns)");
+  if (!(nT == 256 || nT == 512 || nT == 1024 || nT == 2048))
+    return fail(ASH_E_UNSUPPORTED, "nT must be one of 256, 512, 1024, 2048");
+  if (!(N >= 0 && dT > 0 && dz > 0)) return fail(ASH_E_ARG, "N must be >= 0, dT and dz positive");
+  if (scheme < ASH_NL_NONE || scheme > ASH_NL_KT_MINMOD) return fail(ASH_E_ARG, "unknown scheme");
+  if (subcycle < ASH_SUBCYCLE_OFF || subcycle > ASH_SUBCYCLE_GLOBAL) return fail(ASH_E_ARG, "unknown subcycle mode");
+  if (subcycle != ASH_SUBCYCLE_OFF && !is_kt(scheme))
+    return fail(ASH_E_ARG, "CFL sub-cycling applies to the KT schemes only");
+  if (subcycle != ASH_SUBCYCLE_OFF && !(cfl_target > 0)) return fail(ASH_E_ARG, "cfl_target must be positive");
+  unsigned* sc = nullptr;
+  int rc = scratch_on_current_device(&sc);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  ColDesc cd{};
+  cd.mode = COL_RAWNL;
+  cd.nl = scheme;
+  cd.N = N;
+  cd.dT = dT;
+  cd.dz = dz;
+  cd.beta_tvd = (float)beta_tvd;
+  cd.subcycle = subcycle;
+  cd.cfl_target = (float)cfl_target;
+  cd.rin = in;
+  cd.rout = out;
+  cd.gmax = sc + NOALLOC_GMAX;
+  cd.subcap = sc + NOALLOC_SUBCAP;
+  // the scratch keeps the previous call's contents: the counter and (outside the
+  // global mode) the reported maximum start at zero
+  hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, sc + NOALLOC_SUBCAP, 0u);
+  if (subcycle == ASH_SUBCYCLE_GLOBAL) {
+    const int nb = (int)((ncols + ABSMAX_COLS - 1) / ABSMAX_COLS);  // <= NOALLOC_PARTIALS (checked above)
+    hipLaunchKernelGGL(k_absmax_cols, dim3(nb), dim3(256), 0, s, in, (long)ncols, (int)nT, sc);
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, s, sc, nb, sc + NOALLOC_GMAX);
+  } else {
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, sc + NOALLOC_GMAX, 0u);
+  }
+  switch (nT / 2) {
+    case 128: launch_rawnl_M<128>(cd, (int)ncols, s); break;
+    case 256: launch_rawnl_M<256>(cd, (int)ncols, s); break;
+    case 512: launch_rawnl_M<512>(cd, (int)ncols, s); break;
+    default: launch_rawnl_M<1024>(cd, (int)ncols, s); break;
+  }
+  if (report_dev) hipLaunchKernelGGL(k_copy_u32x2, dim3(1), dim3(1), 0, s, sc + NOALLOC_GMAX, sc + NOALLOC_SUBCAP, report_dev);
+  return check_launch();
 } catch (...) {
   return caught_exception();
 }

@@ -196,6 +196,21 @@ int ash_max_abs(ash_plan* plan, const float* in_dev, uint32_t* out_bits_dev, voi
  * Synchronises with the device. */
 int ash_plan_subcycle_capped(ash_plan* plan, int64_t* out);
 
+/* ash_nonlinear_step without a plan and without any allocation.  No
+ * counterpart in the reference: for callers that cannot allocate device memory
+ * (code under hipGraph stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The global-max reduction and the sub-step counter use a
+ * fixed device array in the library (64 KiB per GPU, held until the process
+ * ends), so ncols is at most 1048576 (larger: ASH_E_UNSUPPORTED); nT is 256,
+ * 512, 1024 or 2048.  N = beta / (2 c0^3 rho0).  report_dev (optional, 2
+ * words, device memory) receives the float bits of the field's max|u| (global
+ * mode; 0 otherwise) and the number of columns (or fields) whose sub-steps hit
+ * ASH_MAX_SUBSTEPS.  Calls that overlap on one GPU (concurrent streams or
+ * threads) race on that array; calls in stream order do not. */
+int ash_nonlinear_step_noalloc(int64_t ncols, int32_t nT, double N, double dT, const float* in_dev,
+                               float* out_dev, int32_t scheme, double dz, double beta_tvd, int32_t subcycle,
+                               double cfl_target, uint32_t* report_dev, void* stream);
+
 /* _freq_weighted_boundary_step (ASS:658-664) with precalculate_abl_freq. */
 int ash_freq_weighted_boundary_step(ash_plan* plan, const float* in_dev, float* out_dev, void* stream);
 
