@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libaship.so")
 SOURCES = ["aship.hip"]
-HEADERS = ["fft_engine.h", "kernels.h", "column.h", "learned.h"]
+HEADERS = ["fft_engine.h", "kernels.h", "column.h", "generic.h", "learned.h"]
 
 # Packed-FP32 VALU forms are disabled for every HIP translation unit.
 HIPCC_FLAGS = [

@@ -495,7 +495,7 @@ void launch_xinv_L(ash_plan* p, const EpiDesc& e, hipStream_t s) {
 }
 template <int L>
 void launch_ysa_L(ash_plan* p, const ScreenDesc& sd, hipStream_t s) {
-  const size_t lds = (size_t)ysa_rows(L, sd.band, sd.phase != nullptr) * LF * sizeof(float2);
+  const size_t lds = (size_t)ysa_lds_bytes(L, sd.band, sd.phase != nullptr);
   hipLaunchKernelGGL(k_ysa<L>, dim3(p->g.nX * p->g.nfb), dim3(LNT), lds, s, p->spec, p->T, p->g, sd);
 }
 template <int M, int MODE, int NLK>

@@ -202,11 +202,42 @@ __device__ __forceinline__ void column_realspace(float (&u)[S], const LaneCol<S,
       for (int o = 32; o >= LPC; o >>= 1) nsub_w = max(nsub_w, __shfl_xor(nsub_w, o, 64));
       const float dz = (float)(d.dz / nsub);
       const float coef = (float)(Nn / d.dT);
+      // dz N / (2 dT) of one sub-step: u + dz rhs = u + hcdz dG
+      const float hcdz = (float)(0.5 * Nn / d.dT * (d.dz / nsub));
       const int lsrc = L.lane - 1, rsrc = L.lane + 1;
       const bool has_l = L.ln > 0, has_r = L.ln < LPC - 1;
       for (int sub = 0; sub < nsub_w; ++sub) {
-        const bool act = sub < nsub;
-        if constexpr (NLK == NL_KT || NLK == NL_KT_MINMOD) {
+        // (with one column per wave nsub is the wave's count: every sub-step is taken)
+        const bool act = LPC == 64 || sub < nsub;
+        if constexpr ((NLK == NL_KT || NLK == NL_KT_MINMOD) && !BOUNDED) {
+          float h[S + 4], dG[S], s1[S];
+          const bool first = L.ln == 0, last = L.ln == LPC - 1;
+          // halo: 2 samples from each neighbour lane (zero outside the column)
+          {
+            const float a = shf(u[S - 2], lsrc), b = shf(u[S - 1], lsrc);
+            const float cc = shf(u[0], rsrc), dd = shf(u[1], rsrc);
+            h[0] = has_l ? a : 0.f; h[1] = has_l ? b : 0.f;
+            h[S + 2] = has_r ? cc : 0.f; h[S + 3] = has_r ? dd : 0.f;
+#pragma unroll
+            for (int i = 0; i < S; ++i) h[i + 2] = u[i];
+          }
+          kt_dflux_lane<S, NLK == NL_KT>(h, first, last, dG);
+#pragma unroll
+          for (int i = 0; i < S; ++i) s1[i] = act ? fmaf(hcdz, dG[i], u[i]) : u[i];
+          {
+            const float a = shf(s1[S - 2], lsrc), b = shf(s1[S - 1], lsrc);
+            const float cc = shf(s1[0], rsrc), dd = shf(s1[1], rsrc);
+            h[0] = has_l ? a : 0.f; h[1] = has_l ? b : 0.f;
+            h[S + 2] = has_r ? cc : 0.f; h[S + 3] = has_r ? dd : 0.f;
+#pragma unroll
+            for (int i = 0; i < S; ++i) h[i + 2] = s1[i];
+          }
+          kt_dflux_lane<S, NLK == NL_KT>(h, first, last, dG);
+          if (act) {
+#pragma unroll
+            for (int i = 0; i < S; ++i) u[i] = 0.5f * (u[i] + fmaf(hcdz, dG[i], s1[i]));
+          }
+        } else if constexpr (NLK == NL_KT || NLK == NL_KT_MINMOD) {
           float h[S + 4], r0[S], s1[S];
           // halo: 2 samples from each neighbour lane (zero outside the column)
           {
@@ -275,17 +306,27 @@ __device__ __forceinline__ void column_realspace(float (&u)[S], const LaneCol<S,
     if (MODE == COL_RAWNL && live) L.store(d.rout + (long)c * L.nT, u);
   } else if constexpr (FIN) {
     if constexpr (MODE == COL_FINAL_SRC) {
-      for (int k = 0; k < d.n_src; ++k) {
-        const float* sp = d.src[k] + (long)c * L.nT;
+      // loads without a test each (clamped into the column, padding masked after),
+      // so they issue back to back
+      if (live) {
+        for (int k = 0; k < d.n_src; ++k) {
+          const float* sp = d.src[k] + (long)c * L.nT;
+          float sv[S];
 #pragma unroll
-        for (int i = 0; i < S; ++i) u[i] += live ? (L.in(i) ? sp[L.t(i)] : 0.f) : 0.f;
+          for (int i = 0; i < S; ++i) sv[i] = sp[min(L.t(i), L.nT - 1)];
+#pragma unroll
+          for (int i = 0; i < S; ++i) u[i] += L.in(i) ? sv[i] : 0.f;
+        }
       }
       if (d.smask && live && d.smask[c]) {
         const double dl = d.sdelay[c];
         const double w0 = 2.0 * 3.14159265358979323846 * d.s_f0;
         const double cexp = 2.0 * d.s_dur;
+        float tv[S];
 #pragma unroll
-        for (int i = 0; i < S; ++i) u[i] += L.in(i) ? tone_burst(d.taxis[L.t(i)] - dl, w0, d.s_ncyc, cexp, d.s_p0) : 0.f;
+        for (int i = 0; i < S; ++i) tv[i] = d.taxis[min(L.t(i), L.nT - 1)];
+#pragma unroll
+        for (int i = 0; i < S; ++i) u[i] += L.in(i) ? tone_burst(tv[i] - dl, w0, d.s_ncyc, cexp, d.s_p0) : 0.f;
       }
     }
     // (padding samples of a bounded lane are zero: they add nothing to e2 but are
@@ -488,18 +529,28 @@ __global__ __launch_bounds__(256) void k_column(float2* __restrict__ spec, Table
   // irfft of the column's spectrum (bins modified by tof_bin(sel)) into this lane's
   // samples u[0..S), in the ownership of this mode
   auto inv_spec = [&](float (&uo)[S], int sel) {
-    float2 v[E];
+    // all loads of the column first, under one branch (a test per load would
+    // wait for each load before issuing the next)
+    float2 v[E], xn;
+    if (sc_cur) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = sc_cur[ln + LPC * e];
+      xn = sc_cur[M];
+    } else {
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = make_float2(0.f, 0.f);
+      xn = make_float2(0.f, 0.f);
+    }
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int k = ln + LPC * e;
-      float2 x = sc_cur ? sc_cur[k] : make_float2(0.f, 0.f);
+      float2 x = v[e];
       if (k == 0) x.y = 0.f;
       x = tof_bin(sel, k, x, M, d.tofref);
       v[e] = x;
       cb[pidx(k)] = x;
     }
     if (ln == 0) {
-      float2 xn = sc_cur ? sc_cur[M] : make_float2(0.f, 0.f);
       xn.y = 0.f;
       xn = tof_bin(sel, M, xn, M, d.tofref);
       cb[pidx(M)] = xn;

@@ -519,6 +519,11 @@ __global__ __launch_bounds__(LNT) void k_xfft_inv(float2* __restrict__ spec, Tab
 __host__ __device__ inline int ysa_rows(int L, int band, bool screen) {
   return (!screen && 2 * (band + 1) < L) ? 2 * (band + 1) : L;
 }
+// dynamic LDS of k_ysa: the handled entries, then three floats per handled row
+__host__ __device__ inline int ysa_lds_bytes(int L, int band, bool screen) {
+  const int n = ysa_rows(L, band, screen);
+  return n * LF * 8 + n * 3 * 4;
+}
 
 // ---------------------------------------------------------------------------
 // y-sweep without FFT: super-absorbing y pass, DC/Nyquist projection and an
@@ -536,8 +541,10 @@ struct ScreenDesc {
 
 template <int L>
 __global__ __launch_bounds__(LNT) void k_ysa(float2* __restrict__ spec, Tables T, Geom g, ScreenDesc s) {
-  // rows handled x LF entries, sized at launch (ysa_rows): the two boundary bands
-  // need a fraction of a full line, and the smaller buffer lets more blocks share a CU
+  // rows handled x LF entries, sized at launch (ysa_lds_bytes): the two boundary
+  // bands need a fraction of a full line, and the smaller buffer lets more blocks
+  // share a CU.  After the entries: per handled row its super-absorbing weight and
+  // the screen's phase and amplitude.
   extern __shared__ float2 buf[];
   const int tid = threadIdx.x;
   const int x = blockIdx.x / g.nfb, fb = blockIdx.x % g.nfb;
@@ -552,46 +559,64 @@ __global__ __launch_bounds__(LNT) void k_ysa(float2* __restrict__ spec, Tables T
   const bool band = nrows < L;
   auto row_of = [&](int i) { return (band && i > s.band) ? L - nrows + i : i; };
   const int nel = nrows * LF;
-  for (int el = tid; el < nel; el += LNT) {
-    const int line = el % LF, i = el / LF;
-    buf[i * LF + line] = line < nvalid ? base[row_of(i) * sy + line] : make_float2(0.f, 0.f);
-  }
-  __syncthreads();
+  float* s_w = reinterpret_cast<float*>(buf + nel);
+  float* s_ph = s_w + nrows;
+  float* s_am = s_ph + nrows;
   constexpr int PER = (L * LF) / LNT;
+  static_assert(LNT % LF == 0, "the line of a thread must be fixed");
+  const int line = tid % LF, m = f0 + line;
+  const int nq = (nel + LNT - 1) / LNT;  // same for the whole block
+  {
+    // every load is issued before the first LDS store waits for one; rows past
+    // nrows are clamped to a handled row (the line always lies inside the
+    // spectrum row: f0 + LF <= MP)
+    float2 v[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q)
+      if (q < nq) v[q] = base[row_of(min((tid + q * LNT) / LF, nrows - 1)) * sy + line];
+    for (int i = tid; i < nrows; i += LNT) {
+      const int pos = row_of(i);
+      s_w[i] = s.sa ? T.sa_wy[pos] : 0.f;
+      if (s.phase) {
+        const long idx = (long)x * g.nY + pos;
+        s_ph[i] = s.phase[idx];
+        s_am[i] = s.amp ? fminf(fmaxf(s.amp[idx], 1e-6f), 1.f) : 1.f;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int el = tid + q * LNT;
+      if (q < nq && el < nel) buf[el] = line < nvalid ? v[q] : make_float2(0.f, 0.f);
+    }
+  }
+  const float cw = s.sa ? T.sa_cw[min(m, g.MP - 1)] : 0.f;
+  const float fs = (float)m / s.f0b;
+  __syncthreads();
   float2 res[PER];
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
     const int el = tid + q * LNT;
     if (el >= nel) break;
-    const int line = el % LF, i = el / LF;
+    const int i = el / LF;
     const int pos = row_of(i);
-    const int m = f0 + line;
     float2 u = buf[i * LF + line];
-    if (s.sa) {
-      const float w = T.sa_wy[pos];
-      if (w != 0.f) {
-        float2 d;
-        if (pos == 0) d = cscale(csub(buf[(i + 1) * LF + line], u), s.invdy);
-        else if (pos == L - 1) d = cscale(csub(u, buf[(i - 1) * LF + line]), s.invdy);
-        else d = cscale(csub(buf[(i + 1) * LF + line], buf[(i - 1) * LF + line]), s.inv2dy);
-        const float cw = T.sa_cw[min(m, g.MP - 1)];
-        const float2 cd = make_float2(cw * d.y, -cw * d.x);
-        const float2 pin = pos < L / 2 ? cscale(cadd(u, cd), 0.5f) : cscale(csub(u, cd), 0.5f);
-        u = csub(u, cscale(pin, s.sa_strength * w));
-      }
+    const float w = s_w[i];
+    if (w != 0.f) {
+      float2 d;
+      if (pos == 0) d = cscale(csub(buf[(i + 1) * LF + line], u), s.invdy);
+      else if (pos == L - 1) d = cscale(csub(u, buf[(i - 1) * LF + line]), s.invdy);
+      else d = cscale(csub(buf[(i + 1) * LF + line], buf[(i - 1) * LF + line]), s.inv2dy);
+      const float2 cd = make_float2(cw * d.y, -cw * d.x);
+      const float2 pin = pos < L / 2 ? cscale(cadd(u, cd), 0.5f) : cscale(csub(u, cd), 0.5f);
+      u = csub(u, cscale(pin, s.sa_strength * w));
     }
     if (m == 0 || m == g.nyq) u.y = 0.f;
     if (s.phase) {
-      const long idx = (long)x * g.nY + pos;
-      const float fs = (float)m / s.f0b;
-      const float th = s.phase[idx] * fs;
+      const float th = s_ph[i] * fs;
       float sn, cs;
       sincosf(th, &sn, &cs);
       u = cmul(u, make_float2(cs, -sn));
-      if (s.amp) {
-        const float a = fminf(fmaxf(s.amp[idx], 1e-6f), 1.f);
-        u = cscale(u, powf(a, fs));
-      }
+      if (s.amp) u = cscale(u, powf(s_am[i], fs));
       if (m == 0 || m == g.nyq) u.y = 0.f;
     }
     res[q] = u;
@@ -600,8 +625,7 @@ __global__ __launch_bounds__(LNT) void k_ysa(float2* __restrict__ spec, Tables T
   for (int q = 0; q < PER; ++q) {
     const int el = tid + q * LNT;
     if (el >= nel) break;
-    const int line = el % LF, i = el / LF;
-    if (line < nvalid) base[row_of(i) * sy + line] = res[q];
+    if (line < nvalid) base[row_of(el / LF) * sy + line] = res[q];
   }
 }
 
@@ -700,64 +724,92 @@ __device__ __forceinline__ float2 tof_bin(int sel, int k, float2 x, int nyq, con
   return x;
 }
 
-// MC / minmod slope of cell i from its neighbours (ASS:727-755); float32 literal ops
 __device__ __forceinline__ float sgnf(float a) { return a > 0.f ? 1.f : (a < 0.f ? -1.f : 0.f); }
-__device__ __forceinline__ float minmod2f(float a, float b) {
-  return (a * b > 0.f) ? sgnf(a) * fminf(fabsf(a), fabsf(b)) : 0.f;
-}
-__device__ __forceinline__ float minmod3f(float a, float b, float c) {
-  const bool same = (a * b > 0.f) && (b * c > 0.f);
-  const float m = fminf(fminf(fabsf(a), fabsf(b)), fabsf(c));
-  return same ? sgnf(a) * m : 0.f;
-}
+
+// Half the limited slope of a cell, 0.5 sigma, from its differences dm = u_i -
+// u_{i-1} and dp = u_{i+1} - u_i (ASS:727-755).  minmod(a, b) (a b > 0 ? the one
+// of smaller magnitude : 0) is the median of a, b and 0, one v_med3_f32, and
+// minmod(a, b, c) = minmod(a, minmod(b, c)).  With powers of two scaling exactly:
+//   MC:     0.5 minmod(2 dm, (dm + dp)/2, 2 dp) = minmod(dm, (dm + dp)/4, dp)
+//   minmod: 0.5 minmod(dm, dp)
+// (the reference tests a b > 0 on the products, which can underflow to zero for
+// differences below ~1e-19 and return 0 where the median returns the difference).
+__device__ __forceinline__ float med3f(float a, float b, float c) { return __builtin_amdgcn_fmed3f(a, b, c); }
 template <bool MC>
-__device__ __forceinline__ float slope(float um, float u0, float up) {
-  const float dm = u0 - um, dp = up - u0;
-  if (MC) return minmod3f(2.f * dm, 0.5f * (dm + dp), 2.f * dp);
-  return minmod2f(dm, dp);
-}
-// KT central-upwind flux at an interface (ASS:771-793),
-//   (a+ f- - a- f+ - a+ a- (u+ - u-)) / (a+ - a-),  f(u) = -u^2/2,
-//   a+ = max(-u-, -u+, 0), a- = min(-u-, -u+, 0).
-// For Burgers the quotient has closed forms: with both speeds of one sign it
-// is the upwind flux (f- or f+); with u- > 0 > u+ it is 1.5 u- u+; with
-// u- < 0 < u+ it is -(u-^2 + 3 u- u+ + u+^2)/2.  The Lax-Friedrichs fallback
-// for a+ - a- < 1e-14 (ASS:792-793) is kept.
-__device__ __forceinline__ float kt_iface(float umi, float upi) {
-  const float ap = fmaxf(fmaxf(-umi, -upi), 0.f);
-  const float am = fminf(fminf(-umi, -upi), 0.f);
-  const float fm = -0.5f * umi * umi;
-  const float fp = -0.5f * upi * upi;
-  if (ap - am < 1e-14f) {
-    return 0.5f * (fm + fp) - 0.5f * fmaxf(fabsf(umi), fabsf(upi)) * (upi - umi);
-  }
-  if (am == 0.f) return fm;
-  if (ap == 0.f) return fp;
-  return umi > 0.f ? 1.5f * umi * upi : -0.5f * fmaf(umi, umi + 3.f * upi, upi * upi);
+__device__ __forceinline__ float half_slope(float dm, float dp) {
+  if constexpr (MC) return med3f(dm, med3f(0.25f * (dm + dp), dp, 0.f), 0.f);
+  return 0.5f * med3f(dm, dp, 0.f);
 }
 
-// Semi-discrete KT rhs for S samples t0..t0+S-1 from u[-2..S+1] (index offset 2).
+// KT central-upwind flux at an interface (ASS:771-793),
+//   F = (a+ f- - a- f+ - a+ a- (u+ - u-)) / (a+ - a-),  f(u) = -u^2/2,
+//   a+ = max(-u-, -u+, 0), a- = min(-u-, -u+, 0),
+// returned as G = -2 F.  For Burgers the quotient has closed forms: u-^2 with
+// both speeds >= 0 (u-, u+ <= 0), u+^2 with both <= 0, -3 u- u+ for u- > 0 > u+
+// and u-^2 + u+^2 + 3 u- u+ for u- < 0 < u+.  With P = min(u-, 0), Q = max(u+, 0)
+// and R = max(u-, 0) min(u+, 0) all four are P^2 + Q^2 + 3 (P Q - R), without a
+// branch or a select.  The Lax-Friedrichs fallback of ASS:792-796 acts only where
+// both states lie below 1e-14 in magnitude; there both fluxes are below 1e-28 and
+// change no single-precision sample through dz N/dT dF, so the closed form stands
+// for it.
+__device__ __forceinline__ float kt_iface_g(float um, float up) {
+  const float P = fminf(um, 0.f), Q = fmaxf(up, 0.f);
+  const float R = fmaxf(um, 0.f) * fminf(up, 0.f);
+  return fmaf(3.f, fmaf(P, Q, -R), fmaf(Q, Q, P * P));
+}
+
+// Semi-discrete KT rhs for S samples t0..t0+S-1 from u[-2..S+1] (index offset 2),
+// for columns whose ends fall anywhere in a lane (the general-length kernel).
 template <int S, bool MC>
 __device__ __forceinline__ void kt_rhs_local(const float* u, int t0, int nT, float coef, float* rhs) {
-  float sig[S + 2];
+  float hs[S + 2];
 #pragma unroll
   for (int i = 0; i < S + 2; ++i) {
     const int t = t0 - 1 + i;
-    sig[i] = (t >= 1 && t <= nT - 2) ? slope<MC>(u[i], u[i + 1], u[i + 2]) : 0.f;
+    hs[i] = (t >= 1 && t <= nT - 2) ? half_slope<MC>(u[i + 1] - u[i], u[i + 2] - u[i + 1]) : 0.f;
   }
-  float fl[S + 1];
+  float G[S + 1];
 #pragma unroll
   for (int i = 0; i < S + 1; ++i) {
     // interface between t = t0-1+i and t0+i ; valid for 0 <= t0-1+i <= nT-2
-    const float um = u[i + 1] + 0.5f * sig[i];
-    const float up = u[i + 2] - 0.5f * sig[i + 1];
-    fl[i] = kt_iface(um, up);
+    G[i] = kt_iface_g(u[i + 1] + hs[i], u[i + 2] - hs[i + 1]);
   }
+  const float hc = 0.5f * coef;
 #pragma unroll
   for (int i = 0; i < S; ++i) {
     const int t = t0 + i;
-    rhs[i] = (t >= 1 && t <= nT - 2) ? -coef * (fl[i + 1] - fl[i]) : 0.f;
+    rhs[i] = (t >= 1 && t <= nT - 2) ? hc * (G[i + 1] - G[i]) : 0.f;
   }
+}
+
+// Flux differences of the semi-discrete KT update for the S contiguous samples of
+// one lane of an FFT column (t0 = S ln, nT = S LPC), from h = u[t0-2 .. t0+S+1]
+// (neighbour samples outside the column are zero).  Output dG[k] = G(k, k+1) -
+// G(k-1, k) with the interface flux F = -G/2, so the rhs of ASS:757-799 is
+// N/(2 dT) dG.  The arithmetic of kt_rhs_local with the differences shared
+// between neighbouring slopes; the slope and the rhs at t = 0 and t = nT-1 are
+// zero (ASS:765-766, 799), and only lanes `first` (ln = 0) and `last`
+// (ln = LPC-1) hold those samples.
+template <int S, bool MC>
+__device__ __forceinline__ void kt_dflux_lane(const float* h, bool first, bool last, float* dG) {
+  float d[S + 3];
+#pragma unroll
+  for (int j = 0; j < S + 3; ++j) d[j] = h[j + 1] - h[j];
+  // half slopes of samples k = -1 .. S (index k + 1)
+  float hs[S + 2];
+#pragma unroll
+  for (int k = 0; k < S + 2; ++k) hs[k] = half_slope<MC>(d[k], d[k + 1]);
+  if (first) hs[1] = 0.f;
+  if (last) hs[S] = 0.f;
+  // G(k, k+1), k = -1 .. S-1 (index k + 1), between the states
+  // u_k + sigma_k/2 and u_{k+1} - sigma_{k+1}/2
+  float G[S + 1];
+#pragma unroll
+  for (int k = 0; k < S + 1; ++k) G[k] = kt_iface_g(h[k + 1] + hs[k], h[k + 2] - hs[k + 1]);
+#pragma unroll
+  for (int k = 0; k < S; ++k) dG[k] = G[k + 1] - G[k];
+  if (first) dG[0] = 0.f;
+  if (last) dG[S - 1] = 0.f;
 }
 
 }  // namespace ash
