@@ -718,7 +718,7 @@ EpiDesc epi_base(const ash_plan* p) {
 
 int net_apply(ash_net* n, float* u, int ncols, double c, double dpdt_max, double dT, int32_t* flag,
               hipStream_t s) {
-  const int grid = std::min(ncols, std::max(n->n_cu, 1));  // one 90-138 KB LDS block per CU
+  const int grid = std::min(ncols, std::max(n->n_cu, 1));  // one ~154 KB LDS block (8 waves) per CU
   const float inv = (float)(1.0 / dT);
   switch (n->nT) {
     case 256: hipLaunchKernelGGL(net::k_net_infer<256>, dim3(grid), dim3(net::INF_THREADS), 0, s, u, ncols, n->Wm, n->params, (float)c, (float)dpdt_max, inv, flag); break;

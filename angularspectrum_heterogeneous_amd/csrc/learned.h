@@ -64,20 +64,27 @@ __device__ __forceinline__ void conv_tile(const u16* in, int p0, int lane, const
 
 // ---------------------------------------------------------------------------
 // Fused inference over real columns R[c][t] (in place), persistent blocks.
-// Each column: A = max|u| and the largest step D = max|u[t+1]-u[t]| in one
-// pass.  A column with D / dT > dpdt_max is flagged (flag[c] = 1) and left to
-// the KT kernel.  A column with c D < NEGLIGIBLE is left unchanged: the Burgers
-// increment of one step is bounded by about c A D (c = dz N / dT), so there it
-// is below 1e-5 of the column's own amplitude.  Otherwise the four layers run
-// on chunks of TC outputs with a 16-sample halo on each side (the receptive
-// field of four 9-tap layers), all activations in LDS; a chunk whose window
-// stays below QUIET * A is left unchanged.
+// Each wave works on its own columns and never waits for another wave (one
+// block barrier, after the weights are staged).  Per column: A = max|u| and the
+// largest step D = max|u[t+1]-u[t]|.  A column with D / dT > dpdt_max is
+// flagged (flag[c] = 1) and left to the KT kernel.  A column with c D <
+// NEGLIGIBLE is left unchanged: the Burgers increment of one step is bounded by
+// about c A D (c = dz N / dT), so there it is below 1e-5 of the column's own
+// amplitude.  Otherwise the four layers run on segments of SEG outputs with a
+// 16-sample halo on each side (the receptive field of four 9-tap layers); a
+// segment whose window stays below QUIET * A is left unchanged.
+//
+// The layers of a segment run in place in the wave's LDS buffer: the tile of
+// positions p0 .. p0+15 of layer j reads the rows of positions p0-4 .. p0+19
+// of layer j-1 and writes its outputs four rows lower, over rows that no later
+// tile of the layer reads (the wave walks the tiles in increasing p0).  With
+// the features of positions t0-16 .. t0+SEG+15 in rows 0 .. SEG+31, layer j
+// position q sits in row q - 4j (position index relative to t0-16).
 // ---------------------------------------------------------------------------
-constexpr int INF_THREADS = 768;
-constexpr int INF_WAVES = INF_THREADS / 64;
-// chunk of outputs per pass: 512 (34/33/33/32 tiles per layer keep the 8 waves
-// busy between the layer barriers), the whole column when it is shorter
-template <int NT_> constexpr int chunk_len() { return NT_ < 512 ? NT_ : 512; }
+constexpr int INF_WAVES = 8;
+constexpr int INF_THREADS = 64 * INF_WAVES;
+constexpr int SEG = 128;                  // outputs per segment
+constexpr int WROWS = SEG + 40;           // rows of a wave's buffer: the window, and the last tiles' overrun
 constexpr float QUIET = 1e-4f;
 constexpr float NEGLIGIBLE = 1e-5f;
 
@@ -86,8 +93,9 @@ constexpr float NEGLIGIBLE = 1e-5f;
 // 16-position tile is one matrix-core product per 16 output channels instead
 // of nine.  W0p[co][K] is the packed weight, lane l's B operand holds
 // K = 8 (l/16) .. +8 of position p0 + l%16: the (x, x^2) pairs of taps
-// 4 (l/16) .. 4 (l/16) + 3 (zero past the last tap).
-__device__ __forceinline__ void conv_first_tile(const u16* in, const u16* W0p, int p0, int lane, f32x4 (&acc)[2]) {
+// 4 (l/16) .. 4 (l/16) + 3 (zero past the last tap).  a0[mt] holds the lane's
+// A fragment W0p[16 mt + l%16][8 (l/16) .. +8).
+__device__ __forceinline__ void conv_first_tile(const u16* in, const bf16x8 (&a0)[2], int p0, int lane, f32x4 (&acc)[2]) {
   const int n = lane & 15, b = lane >> 4;
   uint32_t w[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
@@ -97,33 +105,29 @@ __device__ __forceinline__ void conv_first_tile(const u16* in, const u16* W0p, i
   }
   const bf16x8 bv = __builtin_bit_cast(bf16x8, make_uint4(w[0], w[1], w[2], w[3]));
 #pragma unroll
-  for (int mt = 0; mt < 2; ++mt) {
-    const bf16x8 av = *reinterpret_cast<const bf16x8*>(W0p + (16 * mt + n) * C + 8 * b);
-    acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
-  }
+  for (int mt = 0; mt < 2; ++mt) acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0[mt], bv, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
 }
 
 template <int NT_>
 __global__ __launch_bounds__(INF_THREADS) void k_net_infer(float* __restrict__ R, int ncols, const u16* __restrict__ Wm,
                                                            const float* __restrict__ params, float c, float dpdt_max,
                                                            float inv_dt, int32_t* __restrict__ flag) {
-  constexpr int TC = chunk_len<NT_>(), ROWS = TC + 48;
-  static_assert(NT_ % TC == 0, "column length must be a multiple of the chunk");
-  // layers 1 and 2 whole, the first and the output layer packed
-  __shared__ __align__(16) u16 sW[2 * KW * C * C];
+  static_assert(NT_ % SEG == 0 && NT_ % 64 == 0, "column length must be a multiple of the segment");
+  // hidden layers 1 and 2 (network layers 2 and 3) as [layer][k][co][RS]: rows
+  // padded to 80 bytes, so the sixteen co rows a fragment load touches fall in
+  // distinct banks
+  __shared__ __align__(16) u16 sW[2 * KW * C * RS];
   __shared__ __align__(16) u16 sW0[C * C];
   __shared__ __align__(16) u16 sW3[16 * C];
-  __shared__ __align__(16) u16 buf[2][ROWS * RS];
-  __shared__ float scol[NT_];
+  __shared__ __align__(16) u16 buf[INF_WAVES][WROWS * RS];
   __shared__ float sbias[NLAY][C];
-  __shared__ float sred[2][INF_WAVES];
-  __shared__ float sq[INF_WAVES];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   {
     // Wm is [layer][k][co][ci]
-    const uint4* src = reinterpret_cast<const uint4*>(Wm + W_SIZE);
-    uint4* dst = reinterpret_cast<uint4*>(sW);
-    for (int i = tid; i < 2 * W_SIZE / 8; i += INF_THREADS) dst[i] = src[i];
+    for (int i = tid; i < 2 * KW * C * (C / 8); i += INF_THREADS) {
+      const int row = i / (C / 8), q = i % (C / 8);  // row = (layer, k, co)
+      reinterpret_cast<uint4*>(sW + row * RS)[q] = reinterpret_cast<const uint4*>(Wm + W_SIZE + row * C)[q];
+    }
     for (int i = tid; i < C * C; i += INF_THREADS) {
       const int co = i / C, kk = i % C, k = kk >> 1, ci = kk & 1;
       sW0[i] = kk < 2 * KW ? Wm[(k * C + co) * C + ci] : (u16)0;
@@ -136,103 +140,118 @@ __global__ __launch_bounds__(INF_THREADS) void k_net_infer(float* __restrict__ R
     for (int i = tid; i < NLAY * C; i += INF_THREADS) sbias[i / C][i % C] = params[(i / C) * LAYER_SIZE + W_SIZE + i % C];
   }
   __syncthreads();
-  // the next column is read into registers while this one is computed
-  constexpr int NQ = (NT_ + INF_THREADS - 1) / INF_THREADS;
-  float nx[NQ];
-  auto fetch = [&](int cl) {
-    const float* gg = R + (long)cl * NT_;
+  u16* wb = buf[wave];
+  const int n = lane & 15, g4 = lane >> 4;
+  const float b3 = sbias[3][0];
+  // ReLU(acc + bias) as bf16 into rows p0 + n - shift (positions p0 + n), zero
+  // where the position lies outside the column (the zero padding of the next layer)
+  auto emit = [&](int j, int p0, int t0, int shift, const f32x4 (&acc)[2]) {
+    const int p = p0 + n, t = t0 - 16 + p;
+    const bool inside = t >= 0 && t < NT_;
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      const int i = tid + q * INF_THREADS;
-      nx[q] = i < NT_ ? gg[i] : 0.f;
+    for (int mt = 0; mt < 2; ++mt) {
+      const int co = 16 * mt + 4 * g4;
+      float v[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) v[i] = inside ? fmaxf(acc[mt][i] + sbias[j - 1][co + i], 0.f) : 0.f;
+      *reinterpret_cast<uint2*>(wb + (p - shift) * RS + co) = pack4(v[0], v[1], v[2], v[3]);
     }
   };
-  if (blockIdx.x < ncols) fetch(blockIdx.x);
-  for (int col = blockIdx.x; col < ncols; col += gridDim.x) {
+  const int gwave = blockIdx.x * INF_WAVES + wave, nwaves = gridDim.x * INF_WAVES;
+  for (int col = gwave; col < ncols; col += nwaves) {
     float* g = R + (long)col * NT_;
+    // column statistics: lane l reads samples E l .. E l + E - 1
+    constexpr int E = NT_ / 64;
+    float am = 0.f, dm = 0.f, first = 0.f, last = 0.f;
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      const int i = tid + q * INF_THREADS;
-      if (i < NT_) scol[i] = nx[q];
+    for (int q = 0; q < E / 4; ++q) {
+      const float4 v = reinterpret_cast<const float4*>(g + E * lane)[q];
+      const float w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        am = fmaxf(am, fabsf(w[i]));
+        if (q + i > 0) dm = fmaxf(dm, fabsf(w[i] - last));
+        if (q + i == 0) first = w[i];
+        last = w[i];
+      }
     }
-    __syncthreads();
-    if (col + (int)gridDim.x < ncols) fetch(col + gridDim.x);
-    float am = 0.f, dm = 0.f;
-    for (int i = tid; i < NT_; i += INF_THREADS) {
-      const float v = scol[i];
-      am = fmaxf(am, fabsf(v));
-      if (i + 1 < NT_) dm = fmaxf(dm, fabsf(scol[i + 1] - v));
+    {
+      const float nxt = __shfl(first, lane + 1, 64);  // the next lane's first sample
+      if (lane < 63) dm = fmaxf(dm, fabsf(nxt - last));
     }
     for (int o = 32; o > 0; o >>= 1) {
       am = fmaxf(am, __shfl_xor(am, o, 64));
       dm = fmaxf(dm, __shfl_xor(dm, o, 64));
     }
-    if (lane == 0) { sred[0][wave] = am; sred[1][wave] = dm; }
-    __syncthreads();
-    float A = 0.f, D = 0.f;
+    const bool fallback = dm * inv_dt > dpdt_max;  // wave-uniform
+    if (lane == 0) flag[col] = fallback ? 1 : 0;
+    if (fallback || am == 0.f || c * dm < NEGLIGIBLE) continue;
+    const float A = am, gain = c * A * A;
+    // The window of a segment (samples t0-16 .. t0+SEG+15) is read into registers
+    // while the previous segment is computed, before that segment writes its
+    // outputs: every layer input is an original sample.  The features go to the
+    // rows as bf16 (x, x^2) and the sample itself, in float32, to bytes 64-67 of
+    // its row (the row padding, which no layer writes), where the output update
+    // finds it.
+    constexpr int FQ = (SEG + 32 + 63) / 64;
+    float fnext[FQ];
+    auto fetch = [&](int ts) {
 #pragma unroll
-    for (int w = 0; w < INF_WAVES; ++w) { A = fmaxf(A, sred[0][w]); D = fmaxf(D, sred[1][w]); }
-    const bool fallback = D * inv_dt > dpdt_max;  // block-uniform
-    if (tid == 0) flag[col] = fallback ? 1 : 0;
-    if (fallback || A == 0.f || c * D < NEGLIGIBLE) { __syncthreads(); continue; }
-    const float gain = c * A * A;
-    for (int t0 = 0; t0 < NT_; t0 += TC) {
-      // features: row p is t = t0 - 16 + p
+      for (int q = 0; q < FQ; ++q) {
+        const int t = ts - 16 + lane + 64 * q;
+        fnext[q] = g[min(max(t, 0), NT_ - 1)];  // clamped address; outside the column masked below
+      }
+    };
+    fetch(0);
+    for (int t0 = 0; t0 < NT_; t0 += SEG) {
+      float fcur[FQ];
+#pragma unroll
+      for (int q = 0; q < FQ; ++q) fcur[q] = fnext[q];
+      if (t0 + SEG < NT_) fetch(t0 + SEG);
       float wm = 0.f;
-      for (int p = tid; p < ROWS; p += INF_THREADS) {
-        const int t = t0 - 16 + p;
-        float x = 0.f;
-        if (t >= 0 && t < NT_) x = scol[t] / A;
-        if (p < TC + 32) wm = fmaxf(wm, fabsf(x));  // the window [t0 - 16, t0 + TC + 16)
-        *reinterpret_cast<uint32_t*>(buf[0] + p * RS) = (uint32_t)f2bf(x) | ((uint32_t)f2bf(x * x) << 16);
+#pragma unroll
+      for (int q = 0; q < FQ; ++q) {
+        const int p = lane + 64 * q, t = t0 - 16 + p;
+        if (p < SEG + 32) {
+          const float v = (t >= 0 && t < NT_) ? fcur[q] : 0.f;
+          const float x = v / A;
+          wm = fmaxf(wm, fabsf(x));
+          *reinterpret_cast<uint32_t*>(wb + p * RS) = (uint32_t)f2bf(x) | ((uint32_t)f2bf(x * x) << 16);
+          *reinterpret_cast<float*>(wb + p * RS + C) = v;
+        }
       }
       for (int o = 32; o > 0; o >>= 1) wm = fmaxf(wm, __shfl_xor(wm, o, 64));
-      if (lane == 0) sq[wave] = wm;
-      __syncthreads();
-      float win = 0.f;
-#pragma unroll
-      for (int w = 0; w < INF_WAVES; ++w) win = fmaxf(win, sq[w]);
-      if (win < QUIET) {  // block-uniform; every thread has read sq before the next chunk writes it
-        __syncthreads();
-        continue;
-      }
-      // hidden layers j = 1..3: outputs at rows 4j .. 4j + 16 n - 1
-      auto emit = [&](u16* out, int j, int p0, const f32x4 (&acc)[2]) {
-        const int p = p0 + (lane & 15), t = t0 - 16 + p;
-        const bool inside = t >= 0 && t < NT_;
-#pragma unroll
-        for (int mt = 0; mt < 2; ++mt) {
-          const int co = 16 * mt + 4 * (lane >> 4);
-          float v[4];
-#pragma unroll
-          for (int i = 0; i < 4; ++i) v[i] = inside ? fmaxf(acc[mt][i] + sbias[j - 1][co + i], 0.f) : 0.f;
-          *reinterpret_cast<uint2*>(out + p * RS + co) = pack4(v[0], v[1], v[2], v[3]);
-        }
-      };
+      if (wm < QUIET) continue;  // wave-uniform; the segment stays as it is
+      wave_sync();
+      // layer 1 (two input channels, packed taps): positions 4 .. SEG+27 -> rows 0 ..
       {
-        constexpr int ntile = (TC + 32 - 8 + 15) / 16;
-        for (int tt = wave; tt < ntile; tt += INF_WAVES) {
-          const int p0 = 4 + 16 * tt;
+        bf16x8 a0[2];
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt) a0[mt] = *reinterpret_cast<const bf16x8*>(sW0 + (16 * mt + n) * C + 8 * g4);
+        for (int p0 = 4; p0 < SEG + 28; p0 += 16) {
           f32x4 acc[2];
-          conv_first_tile(buf[0], sW0, p0, lane, acc);
-          emit(buf[1], 1, p0, acc);
+          conv_first_tile(wb, a0, p0, lane, acc);
+          wave_sync();  // every lane has read the rows before any is overwritten
+          emit(1, p0, t0, 4, acc);
+          wave_sync();
         }
-        __syncthreads();
       }
+      // layers 2 and 3: positions 4j .. SEG+31-4j from rows shifted by 4(j-1)
 #pragma unroll 1
       for (int j = 2; j <= 3; ++j) {
-        const u16* in = buf[(j - 1) & 1];
-        u16* out = buf[j & 1];
-        const int first = 4 * j, ntile = (TC + 32 - 8 * j + 15) / 16;
         bf16x8 a[2][KW];
-        load_a<2>(sW + (j - 2) * KW * C * C, lane, a);
-        // the nine B fragments of a tile are read before its products, one wait per tile
-        const u16* rowp = in + ((lane & 15) - HW) * RS + 8 * (lane >> 4);
-        for (int tt = wave; tt < ntile; tt += INF_WAVES) {
-          const int p0 = first + 16 * tt;
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+          for (int k = 0; k < KW; ++k)
+            a[mt][k] = *reinterpret_cast<const bf16x8*>(sW + (((j - 2) * KW + k) * C + 16 * mt + n) * RS + 8 * g4);
+        const int sh_in = 4 * (j - 1);
+        for (int p0 = 4 * j; p0 < SEG + 32 - 4 * j; p0 += 16) {
+          // rows of positions p0-4 .. p0+19 of layer j-1
+          const u16* rowp = wb + (p0 - HW + n - sh_in) * RS + 8 * g4;
           bf16x8 cur[KW];
 #pragma unroll
-          for (int k = 0; k < KW; ++k) cur[k] = *reinterpret_cast<const bf16x8*>(rowp + (p0 + k) * RS);
+          for (int k = 0; k < KW; ++k) cur[k] = *reinterpret_cast<const bf16x8*>(rowp + k * RS);
           f32x4 acc[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
 #pragma unroll
           for (int k = 0; k < KW; ++k) {
@@ -242,49 +261,34 @@ __global__ __launch_bounds__(INF_THREADS) void k_net_infer(float* __restrict__ R
           // scheduling groups: the nine LDS reads, then the eighteen products
           __builtin_amdgcn_sched_group_barrier(0x100, KW, 0);
           __builtin_amdgcn_sched_group_barrier(0x008, 2 * KW, 0);
-          emit(out, j, p0, acc);
+          wave_sync();  // every lane has read the rows before any is overwritten
+          emit(j, p0, t0, 4 * j, acc);
+          wave_sync();
         }
-        __syncthreads();
       }
-      // output layer: rows 16 .. 16 + TC - 1 are t0 .. t0 + TC - 1.  With the
-      // taps as matrix rows, D[k][n] = sum_ci W[0][ci][k] h[p0 - 4 + n][ci] for
-      // 32 consecutive rows n (two products), and output m of the tile is the
-      // diagonal sum over k of D[k][m + k], gathered through LDS (buf[0], free
-      // once the last hidden layer has been read).
+      // output layer: positions 16 .. SEG+15 are t0 .. t0+SEG-1, from the layer-3
+      // rows of positions p0-4 .. p0+19 (row = position - 12), nine taps on the
+      // matrix cores with the single output channel in matrix row 0 (the other
+      // rows of A are zero)
       {
-        const bf16x8 a3 = *reinterpret_cast<const bf16x8*>(sW3 + (lane & 15) * C + 8 * (lane >> 4));
-        const u16* in = buf[1];
-        // per wave: D1 (16 x 16) and the 8 columns of D2 that reach the tile (16 x 8)
-        static_assert(INF_WAVES * 384 * 4 <= ROWS * RS * 2, "diagonal scratch fits in one activation buffer");
-        float* dsc = reinterpret_cast<float*>(buf[0]) + wave * 384;
-        const int n = lane & 15, b = lane >> 4;
-        for (int tt = wave; tt < TC / 16; tt += INF_WAVES) {
-          const int p0 = 16 + 16 * tt;
-          const bf16x8 b1 = *reinterpret_cast<const bf16x8*>(in + (p0 + n - HW) * RS + 8 * b);
-          const bf16x8 b2 = *reinterpret_cast<const bf16x8*>(in + (p0 + n + 16 - HW) * RS + 8 * b);
-          const f32x4 z = {0.f, 0.f, 0.f, 0.f};
-          const f32x4 d1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3, b1, z, 0, 0, 0);
-          const f32x4 d2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3, b2, z, 0, 0, 0);
+        bf16x8 a3[KW];
+        const bf16x8 zero8 = __builtin_bit_cast(bf16x8, make_uint4(0u, 0u, 0u, 0u));
 #pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            dsc[(4 * b + i) * 16 + n] = d1[i];
-            if (n < 8) dsc[256 + (4 * b + i) * 8 + n] = d2[i];
-          }
-          wave_sync();
-          if (b == 0) {
-            float sum = 0.f;
+        for (int k = 0; k < KW; ++k) a3[k] = n == 0 ? *reinterpret_cast<const bf16x8*>(sW3 + k * C + 8 * g4) : zero8;
+        for (int p0 = 16; p0 < SEG + 16; p0 += 16) {
+          const u16* rowp = wb + (p0 - 16 + n) * RS + 8 * g4;
+          bf16x8 cur[KW];
 #pragma unroll
-            for (int k = 0; k < KW; ++k) {
-              const int q = n + k;
-              sum += q < 16 ? dsc[k * 16 + q] : dsc[256 + k * 8 + q - 16];
-            }
-            const int t = t0 + 16 * tt + n;
-            g[t] = scol[t] + gain * (sum + sbias[3][0]);
+          for (int k = 0; k < KW; ++k) cur[k] = *reinterpret_cast<const bf16x8*>(rowp + k * RS);
+          f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int k = 0; k < KW; ++k) acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3[k], cur[k], acc, 0, 0, 0);
+          if (g4 == 0) {
+            const int t = t0 + p0 - 16 + n;
+            g[t] = *reinterpret_cast<const float*>(wb + (p0 + n) * RS + C) + gain * (acc[0] + b3);
           }
-          wave_sync();
         }
       }
-      __syncthreads();
     }
   }
 }

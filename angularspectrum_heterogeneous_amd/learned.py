@@ -126,7 +126,7 @@ class LearnedStep:
         """In place u += c A^2 y on columns with max|du/dt| <= dpdt_max (Pa/s); returns the fallback flags.
 
         Columns with c max|u[t+1] - u[t]| < 1e-5 (a Burgers increment below 1e-5 of their
-        amplitude) and quiet 512-sample chunks (below 1e-4 A) are left unchanged."""
+        amplitude) and quiet 128-sample segments (below 1e-4 A) are left unchanged."""
         if u.device.type != "cuda" or u.dtype != torch.float32 or u.shape[-1] != self.nT or not u.is_contiguous():
             raise ValueError(f"expected a contiguous CUDA float32 tensor with last dimension {self.nT}")
         ncols = u.numel() // self.nT

@@ -181,11 +181,10 @@ def apply(params: np.ndarray, u: np.ndarray, c: float, quant: bool = True, chunk
     has c D < ``negligible`` is unchanged (its Burgers increment, about c A D,
     is below that fraction of A), and an output chunk [t0, t0 + chunk) whose
     window [t0 - halo, t0 + chunk + halo) has max |u / A| below ``quiet`` is
-    unchanged.  The kernel's chunk is 512 samples, the whole column when shorter
-    (``chunk=0``)."""
+    unchanged.  The kernel's segment is 128 samples (``chunk=0``)."""
     u = np.asarray(u, dtype=np.float32)
     if chunk <= 0:
-        chunk = min(u.shape[1], 512)
+        chunk = min(u.shape[1], 128)
     X0, A = features(u, quant)
     _, y = forward(params, X0, quant)
     As = np.where(A > 0, A, np.float32(1.0)).astype(np.float32)

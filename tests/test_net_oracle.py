@@ -88,9 +88,10 @@ def test_apply_leaves_quiet_chunks_unchanged():
     u = np.zeros((2, 1024), np.float32)
     u[:, 560:600] = rng.normal(size=(2, 40)).astype(np.float32)
     out = NO.apply(p, u, 1e-3)
-    # chunk 0 (outputs [0, 512), window [0, 528)) sees only zeros: unchanged
+    # 128-sample segments: only [512, 640) (window [496, 656)) sees the data
     assert np.array_equal(out[:, :512], u[:, :512])
-    assert not np.array_equal(out[:, 512:], u[:, 512:])
+    assert not np.array_equal(out[:, 512:640], u[:, 512:640])
+    assert np.array_equal(out[:, 640:], u[:, 640:])
 
 
 def test_apply_leaves_negligible_columns_unchanged():
