@@ -483,9 +483,9 @@ template <int L>
 void launch_ypass_L(ash_plan* p, const OpDesc& op, hipStream_t s) {
   const dim3 grid((p->g.nX / 2) * (p->g.MP / ypass_lines<L>()));  // one block per (x row pair, f group)
   // the FP64-operator reference runs with the default radix order
-  if (op.precise) hipLaunchKernelGGL((k_ypass<L, true, 0>), grid, dim3(LNT), 0, s, p->spec, p->T, p->g, op);
-  else if (p->rvy) hipLaunchKernelGGL((k_ypass<L, false, 1>), grid, dim3(LNT), 0, s, p->spec, p->T, p->g, op);
-  else hipLaunchKernelGGL((k_ypass<L, false, 0>), grid, dim3(LNT), 0, s, p->spec, p->T, p->g, op);
+  if (op.precise) hipLaunchKernelGGL((k_ypass<L, true, 0>), grid, dim3(ypass_threads<L, 0>()), 0, s, p->spec, p->T, p->g, op);
+  else if (p->rvy) hipLaunchKernelGGL((k_ypass<L, false, 1>), grid, dim3(ypass_threads<L, 1>()), 0, s, p->spec, p->T, p->g, op);
+  else hipLaunchKernelGGL((k_ypass<L, false, 0>), grid, dim3(ypass_threads<L, 0>()), 0, s, p->spec, p->T, p->g, op);
 }
 template <int L, int EPI>
 void launch_xinv_L(ash_plan* p, const EpiDesc& e, hipStream_t s) {

@@ -322,6 +322,10 @@ __global__ __launch_bounds__(LNT) void k_xfft_fwd(float2* __restrict__ spec, Tab
 // f bins per y-pass tile: 8 for the 384-point line, whose 16-bin tile (78 KB of
 // LDS with the operator table) would hold the CU to two blocks
 template <int L> constexpr int ypass_lines() { return L == 384 ? 8 : LF; }
+// threads of a y-pass block: 512 for the 256-point 8*8*4 order, whose every pass
+// has 512 or more butterflies per tile (twice the waves per CU in the same LDS),
+// else LNT
+template <int L, int V> constexpr int ypass_threads() { return (L == 256 && V == 1) ? 512 : LNT; }
 
 // ---------------------------------------------------------------------------
 // y-pass: forward FFT along y, multiply by the visit operator, inverse FFT.
@@ -335,8 +339,9 @@ template <int L> constexpr int ypass_lines() { return L == 384 ? 8 : LF; }
 // (|ky| index, f bin) into an LDS table and applies it to both rows: a
 // quarter of one evaluation per element.
 template <int L, bool PRECISE, int V>
-__global__ __launch_bounds__(LNT) void k_ypass(float2* __restrict__ spec, Tables T, Geom g, OpDesc op) {
+__global__ __launch_bounds__((ypass_threads<L, V>())) void k_ypass(float2* __restrict__ spec, Tables T, Geom g, OpDesc op) {
   constexpr int H = L / 2;
+  constexpr int NTY = ypass_threads<L, V>();
   constexpr int LFY = ypass_lines<L>();
   __shared__ float2 tw[L];
   __shared__ float2 buf[L * LFY];
@@ -350,7 +355,7 @@ __global__ __launch_bounds__(LNT) void k_ypass(float2* __restrict__ spec, Tables
   // lines past the Nyquist bin (padding, f > M) are transformed like the others
   // (no guard branches around the loads and stores): each line is an independent
   // transform, so padding values never reach a valid bin
-  load_tw<L>(tw, T.twy, tid, LNT);
+  load_tw<L>(tw, T.twy, tid, NTY);
   if (tid < LFY) {
     const int m = min(f0 + tid, g.MP - 1);
     const double kt = T.kt[m], ka = T.ka[m];
@@ -366,15 +371,37 @@ __global__ __launch_bounds__(LNT) void k_ypass(float2* __restrict__ spec, Tables
     slc[tid] = lc;
   }
   __syncthreads();
-  static_assert(LNT % LFY == 0, "the line of a thread must be fixed");
+  static_assert(NTY % LFY == 0, "the line of a thread must be fixed");
   const int myline = tid % LFY;
   const LineConst lc = slc[myline];
   using LY = Lay<L, LFY, 0, true>;
+  // The first pass's elements of a row are read into registers ahead of use:
+  // row 0's while the operator table is computed, row 1's while row 0 is
+  // transformed back (when every thread owns whole butterflies of that pass;
+  // otherwise the pass reads memory itself).
+  using P0 = PassR<L, 0, false, V>;
+  constexpr int R0 = P0::R, NB0 = P0::NB, TOT0 = NB0 * LFY;
+  constexpr bool PREF = TOT0 % NTY == 0;
+  constexpr int NPRE = PREF ? (TOT0 / NTY) * R0 : 1;
+  float2 pre[NPRE];
+  auto prefetch = [&](int x) {
+    if constexpr (PREF) {
+      const float2* bx = spec + (long)x * g.nY * g.MP + f0;
+#pragma unroll
+      for (int q = 0; q < TOT0 / NTY; ++q) {
+        int line, j;
+        LY::template map<NB0>(tid + q * NTY, line, j);
+#pragma unroll
+        for (int r = 0; r < R0; ++r) pre[q * R0 + r] = gld(bx, (j + r * NB0) * syb + line * 8u);
+      }
+    }
+  };
+  prefetch(ip);
   {
     const double kx = T.kx[ip];
-    // entry e = tid + LNT k is on line e % LF = myline
+    // entry e = tid + NTY k is on line e % LF = myline
 #pragma unroll 2
-    for (int e = tid; e < H * LFY; e += LNT) {
+    for (int e = tid; e < H * LFY; e += NTY) {
       const double ky = T.ky[e / LFY];
       const double kk = kperp2(kx, ky);
       if constexpr (PRECISE) opt[e] = visit_operator_fp64(op, kk, lc, f0 + myline);
@@ -391,13 +418,16 @@ __global__ __launch_bounds__(LNT) void k_ypass(float2* __restrict__ spec, Tables
     asm volatile("" : "+v"(t));
     const int x = h == 0 ? ip : g.nX - 1 - ip;
     float2* base = spec + (long)x * g.nY * g.MP + f0;
+    int k = 0;  // the first pass asks for its elements in prefetch order
     auto ld = [&](int line, int pos) -> float2 {
-      return gld(base, pos * syb + line * 8u);
+      if constexpr (PREF) return pre[k++];
+      else return gld(base, pos * syb + line * 8u);
     };
     auto st_lds = [&](int line, int pos, float2 v) { buf[LY::idx(line, pos)] = v; };
     // pass 0 stores to LDS only after its barrier, which follows every read of
     // the previous row's last pass
-    fft_run<L, false, LNT, LFY, LY, V>(buf, tw, t, ld, st_lds);
+    fft_run<L, false, NTY, LFY, LY, V>(buf, tw, t, ld, st_lds);
+    if (h == 0) prefetch(g.nX - 1 - ip);
     auto ld_op = [&](int line, int pos) -> float2 {
       const int i = pos < H ? pos : L - 1 - pos;
       return cmul(buf[LY::idx(line, pos)], opt[i * LFY + line]);
@@ -406,7 +436,7 @@ __global__ __launch_bounds__(LNT) void k_ypass(float2* __restrict__ spec, Tables
       gst(base, pos * syb + line * 8u, v);
     };
     __syncthreads();
-    fft_run<L, true, LNT, LFY, LY, V>(buf, tw, t, ld_op, st);
+    fft_run<L, true, NTY, LFY, LY, V>(buf, tw, t, ld_op, st);
   }
 }
 
