@@ -171,6 +171,7 @@ def learned_leg(field_np, kw, dz, K, Wm, P, plain_ms, n_pairs, epochs):
     hist = L.train(net, before, after, c, epochs=epochs, batch=256, lr=2e-3)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
+    progress(f"learned leg: trained in {t_train:.1f} s; plain reference march and threshold sweep")
     del before, after
     torch.cuda.empty_cache()
     ref = angular_spectrum_solve(field_np, SolverParams(**kw), verbose=False)
@@ -183,6 +184,7 @@ def learned_leg(field_np, kw, dz, K, Wm, P, plain_ms, n_pairs, epochs):
 
     sweep = []
     for thr in (1e30, 1.2e13, 6e12, 3e12, 1.5e12):
+        progress(f"learned leg: march at max|dp/dt| threshold {thr:.3g} Pa/s")
         shares = []
 
         def cb(cc, z, f, thr=thr):
@@ -278,6 +280,11 @@ def batch_leg(n, seed, prop, sizes=(128, 192, 256, 384), lengths=(512, 1024, 204
                     "instance's outputs to the host), one graph launch per z-step for all active instances"}
 
 
+def progress(msg: str) -> None:
+    """One progress line on stderr (stdout carries only the result line)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -304,6 +311,7 @@ def main():
               "(--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
 
+    progress("importing torch and the library")
     import numpy as np
     import torch
     from angularspectrum_heterogeneous_amd import _lib, workloads as W
@@ -317,7 +325,9 @@ def main():
     K, Wm, P = args.steps, args.warmup, args.profile_iters
     launch = tuple(int(v) for v in args.launch.split(",")) if args.launch else None
     tune = None if (args.no_autotune or launch) else {}
+    progress(f"timed march: {nX}x{nY}x{nT}, {Wm} warm-up + {K} timed steps")
     elapsed, kernels, fin = timed_march(field_np, params, dz, K, Wm, P, tune=tune, launch=launch)
+    progress(f"timed march done: {elapsed / K * 1e3:.3f} ms per step")
     if launch:
         tune = {"x_radix_order": launch[0], "y_radix_order": launch[1], "column_grid": launch[2]}
     value = vox * K / elapsed
@@ -403,13 +413,17 @@ def main():
         "kernels": kernels,
     }
     if not args.no_batch:
+        progress(f"batch leg: {args.batch_instances} instances")
         result["batch"] = batch_leg(args.batch_instances, 0, 0.02)
+        progress("batch leg: 16 small instances")
         # the small grids, where one instance leaves most of the GPU idle
         result["batch_small"] = batch_leg(16, 1, 0.02, sizes=(128, 192), lengths=(512, 1024))
     if not args.no_learned:
+        progress("learned leg: training data, training, threshold sweep, timed march")
         result["learned"] = learned_leg(field_np, kw, dz, K, Wm, P, ms_step, args.learned_pairs,
                                         args.learned_epochs)
     if not args.no_cpu_baseline:
+        progress("CPU baseline")
         result["cpu_baseline"] = cpu_baseline(args.cpu_steps)
     print(json.dumps(result), flush=True)
 
