@@ -202,40 +202,50 @@ __device__ __forceinline__ void column_realspace(float (&u)[S], const LaneCol<S,
       for (int o = 32; o >= LPC; o >>= 1) nsub_w = max(nsub_w, __shfl_xor(nsub_w, o, 64));
       const float dz = (float)(d.dz / nsub);
       const float coef = (float)(Nn / d.dT);
-      // dz N / (2 dT) of one sub-step: u + dz rhs = u + hcdz dG
-      const float hcdz = (float)(0.5 * Nn / d.dT * (d.dz / nsub));
+      // dz N / (2 dT) of one sub-step: u + dz rhs = u + hcdz dG (no division
+      // for the usual single sub-step)
+      const double hc1 = 0.5 * Nn / d.dT;
+      double hcd = hc1 * d.dz;
+      if (nsub != 1) hcd = hc1 * (d.dz / nsub);
+      const float hcdz = (float)hcd, hh = 0.5f * hcdz;
       const int lsrc = L.lane - 1, rsrc = L.lane + 1;
       const bool has_l = L.ln > 0, has_r = L.ln < LPC - 1;
       for (int sub = 0; sub < nsub_w; ++sub) {
         // (with one column per wave nsub is the wave's count: every sub-step is taken)
         const bool act = LPC == 64 || sub < nsub;
         if constexpr ((NLK == NL_KT || NLK == NL_KT_MINMOD) && !BOUNDED) {
+          // SSP-RK2 as s1 = u + hcdz dG(u), u' = (u + s1 + hcdz dG(s1)) / 2, kept
+          // as w = (u + s1) / 2 = u + hh dG(u) and u' = w + hh dG(s1), hh = hcdz / 2
           float h[S + 4], dG[S], s1[S];
           const bool first = L.ln == 0, last = L.ln == LPC - 1;
-          // halo: 2 samples from each neighbour lane (zero outside the column)
+          // halo: 2 samples from each neighbour lane; past the column's ends the
+          // end sample again, then zero (the end slopes vanish)
           {
             const float a = shf(u[S - 2], lsrc), b = shf(u[S - 1], lsrc);
             const float cc = shf(u[0], rsrc), dd = shf(u[1], rsrc);
-            h[0] = has_l ? a : 0.f; h[1] = has_l ? b : 0.f;
-            h[S + 2] = has_r ? cc : 0.f; h[S + 3] = has_r ? dd : 0.f;
+            h[0] = has_l ? a : 0.f; h[1] = has_l ? b : u[0];
+            h[S + 2] = has_r ? cc : u[S - 1]; h[S + 3] = has_r ? dd : 0.f;
 #pragma unroll
             for (int i = 0; i < S; ++i) h[i + 2] = u[i];
           }
           kt_dflux_lane<S, NLK == NL_KT>(h, first, last, dG);
 #pragma unroll
-          for (int i = 0; i < S; ++i) s1[i] = act ? fmaf(hcdz, dG[i], u[i]) : u[i];
+          for (int i = 0; i < S; ++i) {
+            s1[i] = act ? fmaf(hcdz, dG[i], u[i]) : u[i];
+            u[i] = act ? fmaf(hh, dG[i], u[i]) : u[i];  // w
+          }
           {
             const float a = shf(s1[S - 2], lsrc), b = shf(s1[S - 1], lsrc);
             const float cc = shf(s1[0], rsrc), dd = shf(s1[1], rsrc);
-            h[0] = has_l ? a : 0.f; h[1] = has_l ? b : 0.f;
-            h[S + 2] = has_r ? cc : 0.f; h[S + 3] = has_r ? dd : 0.f;
+            h[0] = has_l ? a : 0.f; h[1] = has_l ? b : s1[0];
+            h[S + 2] = has_r ? cc : s1[S - 1]; h[S + 3] = has_r ? dd : 0.f;
 #pragma unroll
             for (int i = 0; i < S; ++i) h[i + 2] = s1[i];
           }
           kt_dflux_lane<S, NLK == NL_KT>(h, first, last, dG);
           if (act) {
 #pragma unroll
-            for (int i = 0; i < S; ++i) u[i] = 0.5f * (u[i] + fmaf(hcdz, dG[i], s1[i]));
+            for (int i = 0; i < S; ++i) u[i] = fmaf(hh, dG[i], u[i]);
           }
         } else if constexpr (NLK == NL_KT || NLK == NL_KT_MINMOD) {
           float h[S + 4], r0[S], s1[S];
@@ -494,7 +504,7 @@ __global__ __launch_bounds__(256) void k_column(float2* __restrict__ spec, Table
   if constexpr (spectral_in || rewrite) {
     // (COL_RAWNL runs no transform and may be launched without tables)
     for (int i = tid; i < TWT; i += 256) s_tw[i] = T.twc[i];
-    for (int i = tid; i <= M; i += 256) s_twn[i] = T.twgt[i];  // exp(-2 pi i k / nT), k <= M
+    for (int i = tid; i <= M; i += 256) s_twn[i] = cscale(T.twgt[i], 0.5f);  // exp(-2 pi i k / nT) / 2, k <= M
   }
   __syncthreads();
   const float2* twf = s_tw;
@@ -557,13 +567,14 @@ __global__ __launch_bounds__(256) void k_column(float2* __restrict__ spec, Table
     }
     wave_sync();
     // Z[k] = (X_k + conj X_{M-k}) + i W_N^{-k} (X_k - conj X_{M-k}), scaled by 1/N
+    // (the table holds W_N^k / 2: t is half the product, and 2 t is exact)
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int k = ln + LPC * e;
       const float2 a = v[e], b = cconj(cb[pidx(M - k)]);
       const float2 sm = cadd(a, b), df = csub(a, b);
       const float2 t = cmul(cconj(s_twn[k]), df);
-      v[e] = make_float2((sm.x - t.y) * invN, (sm.y + t.x) * invN);
+      v[e] = make_float2(fmaf(-2.f, t.y, sm.x) * invN, fmaf(2.f, t.x, sm.y) * invN);
     }
     wave_sync();
     cfft<M, true>(v, cb, twi, ln);
@@ -652,9 +663,11 @@ __global__ __launch_bounds__(256) void k_column(float2* __restrict__ spec, Table
         const int k = ln + LPC * e;
         const float2 zk = v[e];
         const float2 zm = cconj(cb[pidx(k == 0 ? 0 : M - k)]);
-        const float2 sm = cadd(zk, zm), df = csub(zk, zm);
+        // X_k = (Z_k + conj Z_{M-k}) / 2 - i W_N^k (Z_k - conj Z_{M-k}) / 2, the
+        // halves folded into the table and the fmas
+        const float2 df = csub(zk, zm);
         const float2 t = cmul(s_twn[k], df);
-        if (live) sc[k] = make_float2(0.5f * (sm.x + t.y), 0.5f * (sm.y - t.x));
+        if (live) sc[k] = make_float2(fmaf(0.5f, zk.x, fmaf(0.5f, zm.x, t.y)), fmaf(0.5f, zk.y, fmaf(0.5f, zm.y, -t.x)));
       }
       if (ln == 0 && live) sc[M] = make_float2(z0.x - z0.y, 0.f);
       wave_sync();

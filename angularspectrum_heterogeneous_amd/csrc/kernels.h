@@ -777,15 +777,18 @@ __device__ __forceinline__ float half_slope(float dm, float dp) {
 // returned as G = -2 F.  For Burgers the quotient has closed forms: u-^2 with
 // both speeds >= 0 (u-, u+ <= 0), u+^2 with both <= 0, -3 u- u+ for u- > 0 > u+
 // and u-^2 + u+^2 + 3 u- u+ for u- < 0 < u+.  With P = min(u-, 0), Q = max(u+, 0)
-// and R = max(u-, 0) min(u+, 0) all four are P^2 + Q^2 + 3 (P Q - R), without a
-// branch or a select.  The Lax-Friedrichs fallback of ASS:792-796 acts only where
-// both states lie below 1e-14 in magnitude; there both fluxes are below 1e-28 and
-// change no single-precision sample through dz N/dT dF, so the closed form stands
-// for it.
+// and X = u- u+ for u- < 0 < u+, -u- u+ for u- > 0 > u+, 0 for states of one
+// sign, all four are P^2 + Q^2 + 3 X, without a branch or a select.  X is
+// copysign(min(u- u+, 0), u-): the product is negative exactly when the signs
+// differ, and its magnitude takes the sign of u-.  (X equals P Q - R with
+// R = max(u-, 0) min(u+, 0), bit for bit: both round the one product u- u+.)
+// The Lax-Friedrichs fallback of ASS:792-796 acts only where both states lie
+// below 1e-14 in magnitude; there both fluxes are below 1e-28 and change no
+// single-precision sample through dz N/dT dF, so the closed form stands for it.
 __device__ __forceinline__ float kt_iface_g(float um, float up) {
   const float P = fminf(um, 0.f), Q = fmaxf(up, 0.f);
-  const float R = fmaxf(um, 0.f) * fminf(up, 0.f);
-  return fmaf(3.f, fmaf(P, Q, -R), fmaf(Q, Q, P * P));
+  const float X = copysignf(fminf(um * up, 0.f), um);
+  return fmaf(3.f, X, fmaf(Q, Q, P * P));
 }
 
 // Semi-discrete KT rhs for S samples t0..t0+S-1 from u[-2..S+1] (index offset 2),
@@ -813,13 +816,16 @@ __device__ __forceinline__ void kt_rhs_local(const float* u, int t0, int nT, flo
 }
 
 // Flux differences of the semi-discrete KT update for the S contiguous samples of
-// one lane of an FFT column (t0 = S ln, nT = S LPC), from h = u[t0-2 .. t0+S+1]
-// (neighbour samples outside the column are zero).  Output dG[k] = G(k, k+1) -
-// G(k-1, k) with the interface flux F = -G/2, so the rhs of ASS:757-799 is
-// N/(2 dT) dG.  The arithmetic of kt_rhs_local with the differences shared
-// between neighbouring slopes; the slope and the rhs at t = 0 and t = nT-1 are
-// zero (ASS:765-766, 799), and only lanes `first` (ln = 0) and `last`
-// (ln = LPC-1) hold those samples.
+// one lane of an FFT column (t0 = S ln, nT = S LPC), from h = u[t0-2 .. t0+S+1].
+// Output dG[k] = G(k, k+1) - G(k-1, k) with the interface flux F = -G/2, so the
+// rhs of ASS:757-799 is N/(2 dT) dG.  The arithmetic of kt_rhs_local with the
+// differences shared between neighbouring slopes.  The slope and the rhs at
+// t = 0 and t = nT-1 are zero (ASS:765-766, 799); only lanes `first` (ln = 0)
+// and `last` (ln = LPC-1) hold those samples.  The caller gives those lanes the
+// end sample again as the first sample past the column's end (h[1] = u(0) on
+// `first`, h[S+2] = u(nT-1) on `last`): the difference across the end is then
+// zero, and a limited slope with one zero difference is zero, so the end slopes
+// vanish without a select.  The rhs at the ends is masked here.
 template <int S, bool MC>
 __device__ __forceinline__ void kt_dflux_lane(const float* h, bool first, bool last, float* dG) {
   float d[S + 3];
@@ -829,8 +835,6 @@ __device__ __forceinline__ void kt_dflux_lane(const float* h, bool first, bool l
   float hs[S + 2];
 #pragma unroll
   for (int k = 0; k < S + 2; ++k) hs[k] = half_slope<MC>(d[k], d[k + 1]);
-  if (first) hs[1] = 0.f;
-  if (last) hs[S] = 0.f;
   // G(k, k+1), k = -1 .. S-1 (index k + 1), between the states
   // u_k + sigma_k/2 and u_{k+1} - sigma_{k+1}/2
   float G[S + 1];
