@@ -85,6 +85,13 @@ struct CDir {
 
 template <int M>
 constexpr int col_tw_total() { return CDir<M, false>::total() + CDir<M, true>::total(); }
+// A radix list that reads the same reversed gives the inverse transform the
+// forward table (the direction only conjugates at use): one copy in LDS.
+template <int M>
+constexpr bool col_tw_shared() {
+  return CDir<M, false>::rad(0) == CDir<M, true>::rad(0) && CDir<M, false>::rad(1) == CDir<M, true>::rad(1) &&
+         CDir<M, false>::rad(2) == CDir<M, true>::rad(2);
+}
 
 // One Stockham pass of a column transform; data in registers v[e] at position ln + LPC e.
 template <int M, int P, bool INV>
@@ -491,9 +498,9 @@ __global__ __launch_bounds__(256) void k_column(float2* __restrict__ spec, Table
   using C = ColCfg<M>;
   constexpr int E = C::E, LPC = C::LPC, CPW = C::CPW, S = C::S, NT_ = 2 * M;
   constexpr int TWF = CDir<M, false>::total();
-  constexpr int TWT = col_tw_total<M>();
+  constexpr int TWT = col_tw_shared<M>() ? TWF : col_tw_total<M>();
   __shared__ float2 s_tw[TWT];
-  __shared__ float2 s_twn[M + 1];
+  __shared__ float2 s_twn[M / 2 + 1];  // W_N^k / 2, k <= M/2 (the rest by symmetry)
   __shared__ float2 s_buf[C::WPB * CPW * C::LDSN];
   __shared__ float s_red[C::WPB];
   const int tid = threadIdx.x;
@@ -504,11 +511,22 @@ __global__ __launch_bounds__(256) void k_column(float2* __restrict__ spec, Table
   if constexpr (spectral_in || rewrite) {
     // (COL_RAWNL runs no transform and may be launched without tables)
     for (int i = tid; i < TWT; i += 256) s_tw[i] = T.twc[i];
-    for (int i = tid; i <= M; i += 256) s_twn[i] = cscale(T.twgt[i], 0.5f);  // exp(-2 pi i k / nT) / 2, k <= M
+    for (int i = tid; i <= M / 2; i += 256) s_twn[i] = cscale(T.twgt[i], 0.5f);  // exp(-2 pi i k / nT) / 2
   }
   __syncthreads();
   const float2* twf = s_tw;
-  const float2* twi = s_tw + TWF;
+  const float2* twi = col_tw_shared<M>() ? s_tw : s_tw + TWF;
+  // W_N^k / 2 at bin k = ln + LPC e: bins of the lower half (e < E/2) from the
+  // table, the upper half from W_N^k = -conj(W_N^(M-k)), M-k <= M/2 (e is a
+  // compile-time index, so no select).  M-k is addressed from its smallest
+  // value up, so every read is one base register plus a constant offset.
+  const float2* twlo = s_twn + ln;
+  const float2* twhi = s_twn + (M - ln - LPC * (E - 1));
+  auto twn = [&](int e) -> float2 {
+    if (e < E / 2) return twlo[LPC * e];
+    const float2 w = twhi[LPC * (E - 1 - e)];
+    return make_float2(-w.x, w.y);
+  };
   float2* cb = s_buf + (wave * CPW + cw) * C::LDSN;
   const int ncols = g.nX * g.nY;
   const int per_iter = C::WPB * CPW;
@@ -573,7 +591,7 @@ __global__ __launch_bounds__(256) void k_column(float2* __restrict__ spec, Table
       const int k = ln + LPC * e;
       const float2 a = v[e], b = cconj(cb[pidx(M - k)]);
       const float2 sm = cadd(a, b), df = csub(a, b);
-      const float2 t = cmul(cconj(s_twn[k]), df);
+      const float2 t = cmul(cconj(twn(e)), df);
       v[e] = make_float2(fmaf(-2.f, t.y, sm.x) * invN, fmaf(2.f, t.x, sm.y) * invN);
     }
     wave_sync();
@@ -666,7 +684,7 @@ __global__ __launch_bounds__(256) void k_column(float2* __restrict__ spec, Table
         // X_k = (Z_k + conj Z_{M-k}) / 2 - i W_N^k (Z_k - conj Z_{M-k}) / 2, the
         // halves folded into the table and the fmas
         const float2 df = csub(zk, zm);
-        const float2 t = cmul(s_twn[k], df);
+        const float2 t = cmul(twn(e), df);
         if (live) sc[k] = make_float2(fmaf(0.5f, zk.x, fmaf(0.5f, zm.x, t.y)), fmaf(0.5f, zk.y, fmaf(0.5f, zm.y, -t.x)));
       }
       if (ln == 0 && live) sc[M] = make_float2(z0.x - z0.y, 0.f);
