@@ -146,6 +146,40 @@ __device__ __forceinline__ void cfft(float2* v, float2* cb, const float2* twdir,
 // shuffle within the wave from lane src (caller masks column edges)
 __device__ __forceinline__ float shf(float x, int src) { return __shfl(x, src, 64); }
 
+// Cross-lane moves by DPP (no LDS traffic): x from the lane that CTRL names, in
+// the rows RMASK selects; lanes without a source (or outside RMASK) take id.
+template <int CTRL, int RMASK>
+__device__ __forceinline__ float dpp_f(float x, float id) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(id), __float_as_int(x), CTRL, RMASK, 0xF, false));
+}
+template <int CTRL, int RMASK>
+__device__ __forceinline__ double dpp_d0(double x) {  // identity 0
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, RMASK, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, RMASK, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+template <int CTRL, int RMASK>
+__device__ __forceinline__ void col_red_step(float& mn, float& mx, double& e2) {
+  mn = fminf(mn, dpp_f<CTRL, RMASK>(mn, INFINITY));
+  mx = fmaxf(mx, dpp_f<CTRL, RMASK>(mx, -INFINITY));
+  e2 += dpp_d0<CTRL, RMASK>(e2);
+}
+// Minimum, maximum and sum over the LPC lanes of each column (aligned groups of
+// 16, 32 or 64 lanes): an inclusive scan along each 16-lane row by shifts of 1,
+// 2, 4 and 8 lanes, then lane 15 of each row broadcast into the next row and
+// lane 31 into rows 2 and 3.  The column's result lands in its last lane
+// (ln = LPC - 1); the other lanes hold partial results.
+template <int LPC>
+__device__ __forceinline__ void col_reduce_last(float& mn, float& mx, double& e2) {
+  static_assert(LPC == 16 || LPC == 32 || LPC == 64, "columns of 16, 32 or 64 lanes");
+  col_red_step<0x111, 0xF>(mn, mx, e2);  // row_shr:1
+  col_red_step<0x112, 0xF>(mn, mx, e2);  // row_shr:2
+  col_red_step<0x114, 0xF>(mn, mx, e2);  // row_shr:4
+  col_red_step<0x118, 0xF>(mn, mx, e2);  // row_shr:8
+  if constexpr (LPC >= 32) col_red_step<0x142, 0xA>(mn, mx, e2);  // row_bcast:15 into rows 1 and 3
+  if constexpr (LPC == 64) col_red_step<0x143, 0xC>(mn, mx, e2);  // row_bcast:31 into rows 2 and 3
+}
+
 // Lane view of one column: this lane's S real samples u[i] sit at t = t(i).
 // STRIDED: the FFT's ownership (pairs 2(ln + LPC e), +1); else contiguous
 // samples t0 .. t0 + S - 1.  BOUNDED: the column has nT <= LPC S samples and
@@ -356,13 +390,11 @@ __device__ __forceinline__ void column_realspace(float (&u)[S], const LaneCol<S,
       mx = fmaxf(mx, L.in(i) ? u[i] : -INFINITY);
       e2 += (double)u[i] * (double)u[i];
     }
-    for (int o = LPC / 2; o > 0; o >>= 1) {
-      mn = fminf(mn, __shfl_xor(mn, o, 64));
-      mx = fmaxf(mx, __shfl_xor(mx, o, 64));
-      e2 += __shfl_xor(e2, o, 64);
-    }
+    // (the column's totals in its last lane; wmax below takes the block's maximum
+    // over every lane, partial results included)
+    col_reduce_last<LPC>(mn, mx, e2);
     if (live && row < d.max_rows) {
-      if (L.ln == 0) {
+      if (L.ln == LPC - 1) {
         const long o = (long)row * ncols + c;
         const double ib = d.ibefore[c];
         d.pnp[o] = mn;
