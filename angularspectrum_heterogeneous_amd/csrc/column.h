@@ -158,6 +158,28 @@ __device__ __forceinline__ double dpp_d0(double x) {  // identity 0
   const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, RMASK, 0xF, false);
   return __hiloint2double(hi, lo);
 }
+// KT halo of a lane, h[0..1] and h[S+2..S+3] around its samples x[0..S): the
+// previous lane's last two samples and the next lane's first two; past the
+// column's ends the end sample again, then zero (the end slopes vanish).  With
+// one column per wave the neighbours come by DPP wave shifts and the wave's end
+// lanes keep the fallback; narrower columns shuffle and mask.
+template <int S, int LPC>
+__device__ __forceinline__ void kt_halo(float* h, const float* x, int lsrc, int rsrc, bool has_l, bool has_r) {
+  if constexpr (LPC == 64) {
+    h[0] = dpp_f<0x138, 0xF>(x[S - 2], 0.f);       // wave_shr:1, lane i from lane i-1
+    h[1] = dpp_f<0x138, 0xF>(x[S - 1], x[0]);
+    h[S + 2] = dpp_f<0x130, 0xF>(x[0], x[S - 1]);  // wave_shl:1, lane i from lane i+1
+    h[S + 3] = dpp_f<0x130, 0xF>(x[1], 0.f);
+  } else {
+    const float a = shf(x[S - 2], lsrc), b = shf(x[S - 1], lsrc);
+    const float cc = shf(x[0], rsrc), dd = shf(x[1], rsrc);
+    h[0] = has_l ? a : 0.f; h[1] = has_l ? b : x[0];
+    h[S + 2] = has_r ? cc : x[S - 1]; h[S + 3] = has_r ? dd : 0.f;
+  }
+#pragma unroll
+  for (int i = 0; i < S; ++i) h[i + 2] = x[i];
+}
+
 template <int CTRL, int RMASK>
 __device__ __forceinline__ void col_red_step(float& mn, float& mx, double& e2) {
   mn = fminf(mn, dpp_f<CTRL, RMASK>(mn, INFINITY));
@@ -259,30 +281,14 @@ __device__ __forceinline__ void column_realspace(float (&u)[S], const LaneCol<S,
           // as w = (u + s1) / 2 = u + hh dG(u) and u' = w + hh dG(s1), hh = hcdz / 2
           float h[S + 4], dG[S], s1[S];
           const bool first = L.ln == 0, last = L.ln == LPC - 1;
-          // halo: 2 samples from each neighbour lane; past the column's ends the
-          // end sample again, then zero (the end slopes vanish)
-          {
-            const float a = shf(u[S - 2], lsrc), b = shf(u[S - 1], lsrc);
-            const float cc = shf(u[0], rsrc), dd = shf(u[1], rsrc);
-            h[0] = has_l ? a : 0.f; h[1] = has_l ? b : u[0];
-            h[S + 2] = has_r ? cc : u[S - 1]; h[S + 3] = has_r ? dd : 0.f;
-#pragma unroll
-            for (int i = 0; i < S; ++i) h[i + 2] = u[i];
-          }
+          kt_halo<S, LPC>(h, u, lsrc, rsrc, has_l, has_r);
           kt_dflux_lane<S, NLK == NL_KT>(h, first, last, dG);
 #pragma unroll
           for (int i = 0; i < S; ++i) {
             s1[i] = act ? fmaf(hcdz, dG[i], u[i]) : u[i];
             u[i] = act ? fmaf(hh, dG[i], u[i]) : u[i];  // w
           }
-          {
-            const float a = shf(s1[S - 2], lsrc), b = shf(s1[S - 1], lsrc);
-            const float cc = shf(s1[0], rsrc), dd = shf(s1[1], rsrc);
-            h[0] = has_l ? a : 0.f; h[1] = has_l ? b : s1[0];
-            h[S + 2] = has_r ? cc : s1[S - 1]; h[S + 3] = has_r ? dd : 0.f;
-#pragma unroll
-            for (int i = 0; i < S; ++i) h[i + 2] = s1[i];
-          }
+          kt_halo<S, LPC>(h, s1, lsrc, rsrc, has_l, has_r);
           kt_dflux_lane<S, NLK == NL_KT>(h, first, last, dG);
           if (act) {
 #pragma unroll
