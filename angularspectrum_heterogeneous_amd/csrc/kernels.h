@@ -478,8 +478,26 @@ __global__ __launch_bounds__(LNT) void k_xfft_inv(float2* __restrict__ spec, Tab
       ty = e.half ? T.tapy_h[y] : T.tapy_f[y];
       tx = e.half ? T.tapx_h : T.tapx_f;
     }
+    // EPI 1: the taper factors of the positions this thread's last pass stores,
+    // read into registers ahead of the transform (when every thread owns whole
+    // butterflies of that pass), so the stores wait on no load
+    using PL = PassR<L, Radix<L, V>::NP - 1, true, V>;
+    constexpr int RL = PL::R, NBL = PL::NB, TOTL = NBL * LF;
+    constexpr bool TPRE = EPI == 1 && TOTL % LNT == 0;
+    float tf[TPRE ? (TOTL / LNT) * RL : 1];
+    if constexpr (TPRE) {
+#pragma unroll
+      for (int q = 0; q < TOTL / LNT; ++q) {
+        int line, j;
+        LY::template map<NBL>(tid + q * LNT, line, j);
+#pragma unroll
+        for (int r = 0; r < RL; ++r) tf[q * RL + r] = tx[out_pos<L, Radix<L, V>::NP - 1, true, V>(j, r)] * ty;
+      }
+    }
+    int k = 0;  // the last pass stores in the order the factors were read
     auto st = [&](int line, int pos, float2 v) {
-      if constexpr (EPI == 1) v = cscale(v, tx[pos] * ty);
+      if constexpr (TPRE) v = cscale(v, tf[k++]);
+      else if constexpr (EPI == 1) v = cscale(v, tx[pos] * ty);
       if (line < nvalid) gst(base, pos * sxb + line * 8u, v);
     };
     fft_run<L, true, LNT, LF, LY, V>(buf, tw, tid, ld, st);
